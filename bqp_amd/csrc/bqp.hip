@@ -1,0 +1,638 @@
+// bqp host runtime and C ABI (include/bqp.h) over the gfx950 kernels.
+//
+// Boundary mirrored: LightyearSession::integrate_variable_final_into
+// (dust_transfer/crates/lightyear_odeint_rs/src/session.rs:2325-2436) and
+// integrate_batch_native_into (.../src/batch.rs:247-390): caller-owned flat
+// buffers, rows independent, failed rows NaN-filled with a status code, the
+// first failed row reported.  The field store mirrors the process-global
+// coefficient pack of .../src/config.rs:24-223, but is per context.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/bqp.h"
+#include "bqp_kernels.hpp"
+
+using namespace bqp;
+
+struct bqp_ctx {
+  bqp_config cfg{};
+  bqp_launch launch{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int cus = 0;
+  // field
+  int nmax = -1, mmax = -1, jmax = -1;
+  double mu = 0.0, radius = 0.0;
+  GravEntry* d_tab = nullptr;
+  GravEntryF* d_tabf = nullptr;
+  size_t n_entries = 0;
+  // scratch (grown on demand, freed by bqp_destroy)
+  unsigned int* d_queue = nullptr;
+  unsigned long long* d_totals = nullptr;
+  int* d_first = nullptr;
+  void* d_stage = nullptr;
+  size_t stage_bytes = 0;
+  void* d_sort = nullptr;
+  size_t sort_bytes = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+};
+
+namespace {
+
+int fail(bqp_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                          \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(ctx, BQP_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int ensure(bqp_ctx* c, void** ptr, size_t* have, size_t need) {
+  if (*have >= need && *ptr) return BQP_OK;
+  if (*ptr) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(*ptr);
+    *ptr = nullptr;
+    *have = 0;
+  }
+  size_t cap = std::max(need, (size_t)4096);
+  HIP_TRY(c, hipMalloc(ptr, cap));
+  *have = cap;
+  return BQP_OK;
+}
+
+bool valid_config(const bqp_config& c, std::string& why) {
+  if (c.method < 0 || c.method > 2) { why = "method must be 0 (dopri5), 1 (dop853) or 2 (rk87)"; return false; }
+  if (c.control < 0 || c.control > 2) { why = "control must be 0, 1 or 2"; return false; }
+  if (c.precision < 0 || c.precision > 1) { why = "precision must be 0 (fp64) or 1 (split)"; return false; }
+  if (c.drag < 0 || c.drag > 2) { why = "drag must be 0, 1 or 2"; return false; }
+  if (c.control == BQP_CONTROL_SCALED &&
+      !(c.rtol >= 0.0 && c.atol_pos >= 0.0 && c.atol_vel >= 0.0 && (c.rtol > 0 || (c.atol_pos > 0 && c.atol_vel > 0)))) {
+    why = "scaled control needs rtol >= 0, atol >= 0, not all zero";
+    return false;
+  }
+  if (c.control != BQP_CONTROL_SCALED && !(c.eps > 0.0 && std::isfinite(c.eps))) {
+    why = "absolute/compat control needs eps > 0";
+    return false;
+  }
+  if (!std::isfinite(c.omega) || !std::isfinite(c.theta0) || !std::isfinite(c.r_atm) || !(c.r_min >= 0.0)) {
+    why = "omega, theta0, r_atm must be finite and r_min >= 0";
+    return false;
+  }
+  return true;
+}
+
+// ---------------------------------------------------------------- packing
+
+// Builds the regrouped entry table (see bqp_device.hpp) from normalised
+// coefficients; GM/R^2 is folded into the contribution coefficients.
+void pack_field(int N, int M, const double* C, const double* S, double mu, double R,
+                std::vector<GravEntry>& out, int& jmax) {
+  jmax = std::min(N + 1, M + 1);
+  const double gm = mu / (R * R);
+  auto tri = [](int n, int m) { return n * (n + 1) / 2 + m; };
+  auto cc = [&](int n, int m) -> double {
+    if (n < 0 || n > N || m < 0 || m > n || m > M) return 0.0;
+    return C[tri(n, m)];
+  };
+  auto ss = [&](int n, int m) -> double {
+    if (n < 0 || n > N || m < 1 || m > n || m > M) return 0.0;
+    return S[tri(n, m)];
+  };
+  auto kk = [](int n) { return (2.0 * n + 1.0) / (2.0 * n + 3.0); };
+  // f1 for m = 0 is the zonal factor sqrt(k (n+1)(n+2) / 2)
+  auto f1 = [&](int n, int m) {
+    return m == 0 ? std::sqrt(kk(n) * (n + 1.0) * (n + 2.0) / 2.0)
+                  : 0.5 * std::sqrt(kk(n) * (n + m + 1.0) * (n + m + 2.0));
+  };
+  auto f2 = [&](int n, int m) {
+    return 0.5 * std::sqrt(kk(n) * (n - m + 1.0) * (n - m + 2.0)) * (m == 1 ? std::sqrt(2.0) : 1.0);
+  };
+  auto f3 = [&](int n, int m) { return std::sqrt(kk(n) * (n + m + 1.0) * (n - m + 1.0)); };
+  out.clear();
+  out.reserve((size_t)col_offset(N, jmax + 1));
+  for (int j = 0; j <= jmax; ++j) {
+    const int nlo = j == 0 ? 1 : j;
+    for (int n = nlo; n <= N + 1; ++n) {
+      GravEntry e{};
+      if (n == j) {
+        e.a = (j == 1) ? std::sqrt(3.0) : std::sqrt((2.0 * j + 1.0) / (2.0 * j));
+        e.b = 0.0;
+      } else {
+        const double dn = n, dm = j;
+        e.a = std::sqrt((2 * dn + 1) * (2 * dn - 1) / ((dn - dm) * (dn + dm)));
+        e.b = (n >= j + 2) ? std::sqrt((2 * dn + 1) * (dn + dm - 1) * (dn - dm - 1) /
+                                       ((dn - dm) * (dn + dm) * (2 * dn - 3)))
+                           : 0.0;
+      }
+      const int np = n - 1;  // field degree whose terms read V/W(n, .)
+      double al = 0, ga = 0, be = 0, de = 0, zc = 0, zs = 0;
+      if (j - 1 >= 0 && j - 1 <= np) {
+        al = f1(np, j - 1) * cc(np, j - 1);
+        ga = (j - 1 >= 1) ? f1(np, j - 1) * ss(np, j - 1) : 0.0;
+      }
+      if (j + 1 <= np) {
+        be = f2(np, j + 1) * cc(np, j + 1);
+        de = f2(np, j + 1) * ss(np, j + 1);
+      }
+      if (j <= np) {
+        zc = f3(np, j) * cc(np, j);
+        zs = f3(np, j) * ss(np, j);
+      }
+      e.xv = gm * (be - al);
+      e.xw = gm * (de - ga);
+      e.yv = gm * (ga + de);
+      e.yw = gm * (-al - be);
+      e.zv = gm * (-zc);
+      e.zw = gm * (-zs);
+      out.push_back(e);
+    }
+  }
+}
+
+FieldView field_view(const bqp_ctx* c) {
+  FieldView f;
+  f.tab = c->d_tab;
+  f.tabf = c->d_tabf;
+  f.nmax = c->nmax;
+  f.jmax = c->jmax;
+  f.radius = c->radius;
+  return f;
+}
+
+// ---------------------------------------------------------------- launch
+
+typedef void (*prop_fn)(const PropParams);
+
+template <int M, int P>
+prop_fn pick_g(int G) {
+  switch (G) {
+    case 1: return propagate_kernel<M, P, 1>;
+    case 2: return propagate_kernel<M, P, 2>;
+    case 4: return propagate_kernel<M, P, 4>;
+    default: return nullptr;
+  }
+}
+
+template <int M>
+prop_fn pick_p(int P, int G) {
+  return P == 0 ? pick_g<M, 0>(G) : pick_g<M, 1>(G);
+}
+
+prop_fn pick_kernel(int method, int prec, int G) {
+  switch (method) {
+    case BQP_METHOD_DOPRI5: return pick_p<BQP_METHOD_DOPRI5>(prec, G);
+    case BQP_METHOD_DOP853: return pick_p<BQP_METHOD_DOP853>(prec, G);
+    case BQP_METHOD_RK87: return pick_p<BQP_METHOD_RK87>(prec, G);
+    default: return nullptr;
+  }
+}
+
+int default_g(const bqp_ctx* c, int64_t n) {
+  // Enough groups to put >= 2 waves on every SIMD when the batch allows it.
+  const int64_t groups = (n + 63) / 64;
+  const int64_t simds = (int64_t)c->cus * 4;
+  if (groups >= 2 * simds) return 1;
+  if (groups >= simds) return 2;
+  return 4;
+}
+
+// Runs one propagation over device buffers; fills per-row outputs.
+int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
+  if (c->nmax < 0 || !c->d_tab) return fail(c, BQP_E_NO_FIELD, "no gravity field loaded (call bqp_load_field)");
+  if (n <= 0) return BQP_OK;
+  if (n > 0x7fffffffLL) return fail(c, BQP_E_INVALID, "n exceeds 2^31-1 rows");
+  const bqp_config& cfg = c->cfg;
+  int G = c->launch.waves_per_group > 0 ? c->launch.waves_per_group : default_g(c, n);
+  prop_fn kern = pick_kernel(cfg.method, cfg.precision, G);
+  if (!kern) return fail(c, BQP_E_INVALID, "waves_per_group must be 1, 2 or 4");
+  if (cfg.precision == BQP_PRECISION_SPLIT && !c->d_tabf)
+    return fail(c, BQP_E_NO_FIELD, "split precision needs the FP32 table (reload the field)");
+
+  // optional longest-first order
+  int32_t* d_order = nullptr;
+  const bool sort = c->launch.sort_by_cost != 0 && p.t_eval == nullptr && n > 64;
+  if (sort) {
+    // layout in the sort scratch: keys_in, keys_out, idx_in, idx_out, temp
+    size_t temp = 0;
+    HIP_TRY(c, rocprim::radix_sort_pairs_desc(nullptr, temp, (float*)nullptr, (float*)nullptr,
+                                             (int32_t*)nullptr, (int32_t*)nullptr, (unsigned)n, 0, 32,
+                                             c->stream));
+    const size_t al = 256;
+    auto up = [&](size_t b) { return (b + al - 1) / al * al; };
+    const size_t need = 2 * up(n * sizeof(float)) + 2 * up(n * sizeof(int32_t)) + up(temp);
+    int rc = ensure(c, &c->d_sort, &c->sort_bytes, need);
+    if (rc) return rc;
+    char* base = (char*)c->d_sort;
+    float* kin = (float*)base;
+    float* kout = (float*)(base + up(n * sizeof(float)));
+    int32_t* iin = (int32_t*)(base + 2 * up(n * sizeof(float)));
+    int32_t* iout = (int32_t*)(base + 2 * up(n * sizeof(float)) + up(n * sizeof(int32_t)));
+    void* tmp = base + 2 * up(n * sizeof(float)) + 2 * up(n * sizeof(int32_t));
+    hipLaunchKernelGGL(cost_key_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, p.y0,
+                       p.t0, p.tf, c->mu, (int)n, kin, iin);
+    HIP_TRY(c, hipGetLastError());
+    HIP_TRY(c, rocprim::radix_sort_pairs_desc(tmp, temp, kin, kout, iin, iout, (unsigned)n, 0, 32,
+                                             c->stream));
+    d_order = iout;
+  }
+  p.order = d_order;
+  p.queue = c->d_queue;
+  p.n = (int)n;
+  p.field = field_view(c);
+  p.model.omega = cfg.omega;
+  p.model.theta0 = cfg.theta0;
+  p.model.r_atm = cfg.r_atm;
+  p.model.r_min = cfg.r_min;
+  p.model.drag = cfg.drag;
+  p.control = cfg.control;
+  p.max_steps = (int)std::min<int64_t>(cfg.max_steps > 0 ? cfg.max_steps : 500000, 1 << 29);
+  p.max_rejects = cfg.max_rejects > 0 ? cfg.max_rejects : 50;
+  p.rtol = cfg.rtol;
+  p.atol_pos = cfg.atol_pos;
+  p.atol_vel = cfg.atol_vel;
+  p.eps = cfg.eps;
+  p.h0 = cfg.h0;
+  p.h_min = cfg.h_min;
+  p.h_max = cfg.h_max;
+
+  const int64_t groups = (n + 63) / 64;
+  int blocks = c->launch.blocks;
+  if (blocks <= 0) {
+    int per_cu = 0;
+    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kern, 64 * G, 0));
+    if (per_cu < 1) per_cu = 1;
+    blocks = (int)std::min<int64_t>(groups, (int64_t)per_cu * c->cus);
+  }
+  blocks = (int)std::max<int64_t>(1, std::min<int64_t>(blocks, groups));
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
+  HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * G), 0, c->stream, p);
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+  if (stats || p.status) {
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_totals, 5);
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, c->stream, c->d_first, 0x7fffffff);
+    hipLaunchKernelGGL(row_stats_kernel, dim3(std::min<int64_t>(1024, (n + 255) / 256)), dim3(256), 0,
+                       c->stream, p.status, p.steps, p.evals, p.rejects, (int)n, c->d_totals);
+    hipLaunchKernelGGL(first_failed_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                       p.status, (int)n, c->d_first);
+    HIP_TRY(c, hipGetLastError());
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  unsigned long long tot[5] = {0, 0, 0, 0, 0};
+  int first = 0x7fffffff;
+  HIP_TRY(c, hipMemcpy(tot, c->d_totals, sizeof(tot), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(&first, c->d_first, sizeof(int), hipMemcpyDeviceToHost));
+  if (stats) {
+    float ms = 0.0f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    stats->steps = (int64_t)tot[0];
+    stats->evals = (int64_t)tot[1];
+    stats->rejects = (int64_t)tot[2];
+    stats->rows_ok = (int64_t)tot[3];
+    stats->rows_failed = (int64_t)tot[4];
+    stats->first_failed_row = (tot[4] > 0) ? first : -1;
+    stats->kernel_ms = ms;
+    stats->flops = bqp_flops_per_rhs(c) * (double)tot[1];
+  }
+  if (tot[4] > 0) {
+    char buf[160];
+    snprintf(buf, sizeof(buf), "%llu row(s) failed; first failed row %d", tot[4], first);
+    c->err = buf;
+    return BQP_E_ROWS_FAILED;
+  }
+  return BQP_OK;
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+
+extern "C" {
+
+int32_t bqp_default_config(bqp_config* out) {
+  if (!out) return BQP_E_INVALID;
+  std::memset(out, 0, sizeof(*out));
+  out->method = BQP_METHOD_RK87;
+  out->control = BQP_CONTROL_SCALED;
+  out->precision = BQP_PRECISION_FP64;
+  out->drag = BQP_DRAG_NONE;
+  out->rtol = 1e-12;
+  out->atol_pos = 1e-12 * 6378137.0;
+  out->atol_vel = 1e-12 * std::sqrt(3.986004418e14 / 6378137.0);
+  out->eps = 1e-8;
+  out->max_steps = 500000;
+  out->max_rejects = 50;
+  out->omega = 7.292115e-5;
+  out->theta0 = 0.0;
+  out->r_atm = 6378137.0;
+  out->r_min = 0.0;
+  return BQP_OK;
+}
+
+int32_t bqp_create(const bqp_config* cfg, bqp_ctx** out) {
+  if (!out) return BQP_E_INVALID;
+  *out = nullptr;
+  bqp_config def;
+  bqp_default_config(&def);
+  const bqp_config& use = cfg ? *cfg : def;
+  std::string why;
+  if (!valid_config(use, why)) return BQP_E_INVALID;
+  bqp_ctx* c = new (std::nothrow) bqp_ctx();
+  if (!c) return BQP_E_DEVICE;
+  c->cfg = use;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    delete c;
+    return BQP_E_DEVICE;
+  }
+  c->device = dev;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, dev) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+      hipMalloc(&c->d_queue, 256) != hipSuccess || hipMalloc(&c->d_totals, 256) != hipSuccess ||
+      hipMalloc(&c->d_first, 256) != hipSuccess) {
+    bqp_destroy(c);
+    return BQP_E_DEVICE;
+  }
+  c->cus = prop.multiProcessorCount;
+  c->launch.sort_by_cost = 1;
+  *out = c;
+  return BQP_OK;
+}
+
+int32_t bqp_destroy(bqp_ctx* c) {
+  if (!c) return BQP_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->d_tab);
+  (void)hipFree(c->d_tabf);
+  (void)hipFree(c->d_queue);
+  (void)hipFree(c->d_totals);
+  (void)hipFree(c->d_first);
+  (void)hipFree(c->d_stage);
+  (void)hipFree(c->d_sort);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return BQP_OK;
+}
+
+int32_t bqp_set_config(bqp_ctx* c, const bqp_config* cfg) {
+  if (!c || !cfg) return BQP_E_INVALID;
+  std::string why;
+  if (!valid_config(*cfg, why)) return fail(c, BQP_E_INVALID, why);
+  c->cfg = *cfg;
+  return BQP_OK;
+}
+
+int32_t bqp_set_launch(bqp_ctx* c, const bqp_launch* l) {
+  if (!c || !l) return BQP_E_INVALID;
+  if (l->waves_per_group != 0 && l->waves_per_group != 1 && l->waves_per_group != 2 && l->waves_per_group != 4)
+    return fail(c, BQP_E_INVALID, "waves_per_group must be 0 (auto), 1, 2 or 4");
+  if (l->blocks < 0) return fail(c, BQP_E_INVALID, "blocks must be >= 0");
+  c->launch = *l;
+  return BQP_OK;
+}
+
+const char* bqp_last_error(const bqp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int32_t bqp_load_field(bqp_ctx* c, int32_t nmax, int32_t mmax, const double* cbar, const double* sbar,
+                       double mu, double radius) {
+  if (!c) return BQP_E_INVALID;
+  if (!cbar || !sbar) return fail(c, BQP_E_INVALID, "cbar/sbar must not be NULL");
+  if (nmax < 0 || nmax > 200 || mmax < 0 || mmax > nmax)
+    return fail(c, BQP_E_INVALID, "need 0 <= mmax <= nmax <= 200");
+  if (!(mu > 0.0) || !(radius > 0.0) || !std::isfinite(mu) || !std::isfinite(radius))
+    return fail(c, BQP_E_INVALID, "mu and radius must be finite and > 0");
+  const int nt = (nmax + 1) * (nmax + 2) / 2;
+  for (int i = 0; i < nt; ++i)
+    if (!std::isfinite(cbar[i]) || !std::isfinite(sbar[i]))
+      return fail(c, BQP_E_INVALID, "non-finite coefficient");
+  std::vector<GravEntry> tab;
+  int jmax = 0;
+  pack_field(nmax, mmax, cbar, sbar, mu, radius, tab, jmax);
+  std::vector<GravEntryF> tabf(tab.size());
+  for (size_t i = 0; i < tab.size(); ++i) {
+    const GravEntry& e = tab[i];
+    tabf[i] = GravEntryF{(float)e.a, (float)e.b, (float)e.xv, (float)e.xw,
+                         (float)e.yv, (float)e.yw, (float)e.zv, (float)e.zw};
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->d_tab);
+  (void)hipFree(c->d_tabf);
+  c->d_tab = nullptr;
+  c->d_tabf = nullptr;
+  c->nmax = -1;
+  // two zero entries of padding: the unrolled column loops load entries in pairs
+  tab.push_back(GravEntry{});
+  tab.push_back(GravEntry{});
+  tabf.push_back(GravEntryF{});
+  tabf.push_back(GravEntryF{});
+  HIP_TRY(c, hipMalloc(&c->d_tab, tab.size() * sizeof(GravEntry)));
+  HIP_TRY(c, hipMalloc(&c->d_tabf, tabf.size() * sizeof(GravEntryF)));
+  HIP_TRY(c, hipMemcpy(c->d_tab, tab.data(), tab.size() * sizeof(GravEntry), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMemcpy(c->d_tabf, tabf.data(), tabf.size() * sizeof(GravEntryF), hipMemcpyHostToDevice));
+  c->n_entries = tab.size();
+  c->nmax = nmax;
+  c->mmax = mmax;
+  c->jmax = jmax;
+  c->mu = mu;
+  c->radius = radius;
+  return BQP_OK;
+}
+
+int32_t bqp_gravity_accel_batch_dev(bqp_ctx* c, const double* d_pos, int64_t n, double* d_acc) {
+  if (!c) return BQP_E_INVALID;
+  if (c->nmax < 0) return fail(c, BQP_E_NO_FIELD, "no gravity field loaded (call bqp_load_field)");
+  if (n < 0 || (n > 0 && (!d_pos || !d_acc))) return fail(c, BQP_E_INVALID, "bad buffers");
+  if (n == 0) return BQP_OK;
+  const FieldView f = field_view(c);
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (c->cfg.precision == BQP_PRECISION_SPLIT)
+    hipLaunchKernelGGL(gravity_kernel<1>, dim3(blocks), dim3(256), 0, c->stream, f, d_pos, d_acc, (long long)n);
+  else
+    hipLaunchKernelGGL(gravity_kernel<0>, dim3(blocks), dim3(256), 0, c->stream, f, d_pos, d_acc, (long long)n);
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return BQP_OK;
+}
+
+int32_t bqp_gravity_accel_batch(bqp_ctx* c, const double* pos, int64_t n, double* acc) {
+  if (!c) return BQP_E_INVALID;
+  if (c->nmax < 0) return fail(c, BQP_E_NO_FIELD, "no gravity field loaded (call bqp_load_field)");
+  if (n < 0 || (n > 0 && (!pos || !acc))) return fail(c, BQP_E_INVALID, "bad buffers");
+  if (n == 0) return BQP_OK;
+  const size_t bytes = (size_t)n * 3 * sizeof(double);
+  int rc = ensure(c, &c->d_stage, &c->stage_bytes, 2 * bytes);
+  if (rc) return rc;
+  double* dp = (double*)c->d_stage;
+  double* da = dp + 3 * n;
+  HIP_TRY(c, hipMemcpyAsync(dp, pos, bytes, hipMemcpyHostToDevice, c->stream));
+  rc = bqp_gravity_accel_batch_dev(c, dp, n, da);
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpy(acc, da, bytes, hipMemcpyDeviceToHost));
+  return BQP_OK;
+}
+
+int32_t bqp_propagate_batch_final_dev(bqp_ctx* c, const double* d_y0, const double* d_t0, const double* d_tf,
+                                      const double* d_bc, int64_t n, double* d_yf, int32_t* d_status,
+                                      int32_t* d_steps, int32_t* d_evals, bqp_stats* stats) {
+  if (!c) return BQP_E_INVALID;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (n < 0 || (n > 0 && (!d_y0 || !d_t0 || !d_tf || !d_yf)))
+    return fail(c, BQP_E_INVALID, "y0, t0, tf and yf must not be NULL");
+  if (n == 0) return BQP_OK;
+  // per-row counters live in the staging area when the caller passes none
+  const size_t cnt_bytes = (size_t)n * sizeof(int32_t);
+  int rc = ensure(c, &c->d_stage, &c->stage_bytes, 4 * cnt_bytes);
+  if (rc) return rc;
+  int32_t* s0 = (int32_t*)c->d_stage;
+  PropParams p{};
+  p.y0 = d_y0;
+  p.t0 = d_t0;
+  p.tf = d_tf;
+  p.bc = d_bc;
+  p.yf = d_yf;
+  p.status = d_status ? d_status : s0;
+  p.steps = d_steps ? d_steps : s0 + n;
+  p.evals = d_evals ? d_evals : s0 + 2 * n;
+  p.rejects = s0 + 3 * n;
+  return run_propagation(c, p, n, stats);
+}
+
+int32_t bqp_propagate_batch_final(bqp_ctx* c, const double* y0, const double* t0, const double* tf,
+                                  const double* bc, int64_t n, double* yf, int32_t* row_status,
+                                  int32_t* row_steps, int32_t* row_evals, bqp_stats* stats) {
+  if (!c) return BQP_E_INVALID;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (n < 0 || (n > 0 && (!y0 || !t0 || !tf || !yf)))
+    return fail(c, BQP_E_INVALID, "y0, t0, tf and yf must not be NULL");
+  if (n == 0) return BQP_OK;
+  if (c->nmax < 0) return fail(c, BQP_E_NO_FIELD, "no gravity field loaded (call bqp_load_field)");
+  // staging: y0 | t0 | tf | bc | yf | status | steps | evals | rejects
+  const size_t d6 = (size_t)n * 6 * sizeof(double), d1 = (size_t)n * sizeof(double),
+               i1 = (size_t)n * sizeof(int32_t);
+  const size_t need = 2 * d6 + 3 * d1 + 4 * i1 + 256;
+  int rc = ensure(c, &c->d_stage, &c->stage_bytes, need);
+  if (rc) return rc;
+  char* b = (char*)c->d_stage;
+  double* dy0 = (double*)b;
+  double* dt0 = (double*)(b + d6);
+  double* dtf = (double*)(b + d6 + d1);
+  double* dbc = (double*)(b + d6 + 2 * d1);
+  double* dyf = (double*)(b + d6 + 3 * d1);
+  int32_t* dst = (int32_t*)(b + 2 * d6 + 3 * d1);
+  int32_t* dsp = dst + n;
+  int32_t* dev = dsp + n;
+  int32_t* drj = dev + n;
+  HIP_TRY(c, hipMemcpyAsync(dy0, y0, d6, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(dt0, t0, d1, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(dtf, tf, d1, hipMemcpyHostToDevice, c->stream));
+  if (bc) HIP_TRY(c, hipMemcpyAsync(dbc, bc, d1, hipMemcpyHostToDevice, c->stream));
+  PropParams p{};
+  p.y0 = dy0;
+  p.t0 = dt0;
+  p.tf = dtf;
+  p.bc = bc ? dbc : nullptr;
+  p.yf = dyf;
+  p.status = dst;
+  p.steps = dsp;
+  p.evals = dev;
+  p.rejects = drj;
+  const int rc2 = run_propagation(c, p, n, stats);
+  if (rc2 != BQP_OK && rc2 != BQP_E_ROWS_FAILED) return rc2;
+  HIP_TRY(c, hipMemcpy(yf, dyf, d6, hipMemcpyDeviceToHost));
+  if (row_status) HIP_TRY(c, hipMemcpy(row_status, dst, i1, hipMemcpyDeviceToHost));
+  if (row_steps) HIP_TRY(c, hipMemcpy(row_steps, dsp, i1, hipMemcpyDeviceToHost));
+  if (row_evals) HIP_TRY(c, hipMemcpy(row_evals, dev, i1, hipMemcpyDeviceToHost));
+  return rc2;
+}
+
+int32_t bqp_propagate_batch_sampled(bqp_ctx* c, const double* y0, double t0, const double* t_eval, int64_t m,
+                                    const double* bc, int64_t n, double* out, int32_t* row_status,
+                                    bqp_stats* stats) {
+  if (!c) return BQP_E_INVALID;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (n < 0 || m <= 0 || (n > 0 && (!y0 || !t_eval || !out)))
+    return fail(c, BQP_E_INVALID, "need m >= 1 and non-NULL y0, t_eval, out");
+  if (!std::isfinite(t0)) return fail(c, BQP_E_INVALID, "t0 must be finite");
+  for (int64_t i = 0; i < m; ++i) {
+    if (!std::isfinite(t_eval[i]) || t_eval[i] < t0 || (i > 0 && !(t_eval[i] > t_eval[i - 1])))
+      return fail(c, BQP_E_INVALID, "t_eval must be finite, >= t0 and strictly increasing");
+  }
+  if (n == 0) return BQP_OK;
+  if (c->nmax < 0) return fail(c, BQP_E_NO_FIELD, "no gravity field loaded (call bqp_load_field)");
+  const size_t d6 = (size_t)n * 6 * sizeof(double), d1 = (size_t)n * sizeof(double),
+               dm = (size_t)m * sizeof(double), ds = (size_t)n * m * 6 * sizeof(double),
+               i1 = (size_t)n * sizeof(int32_t);
+  const size_t need = d6 + d1 + dm + ds + 4 * i1 + 512;
+  int rc = ensure(c, &c->d_stage, &c->stage_bytes, need);
+  if (rc) return rc;
+  char* b = (char*)c->d_stage;
+  double* dy0 = (double*)b;
+  double* dbc = (double*)(b + d6);
+  double* dte = (double*)(b + d6 + d1);
+  double* dout = (double*)(b + d6 + d1 + dm);
+  int32_t* dst = (int32_t*)(b + d6 + d1 + dm + ds);
+  int32_t* dsp = dst + n;
+  int32_t* dev = dsp + n;
+  int32_t* drj = dev + n;
+  HIP_TRY(c, hipMemcpyAsync(dy0, y0, d6, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(dte, t_eval, dm, hipMemcpyHostToDevice, c->stream));
+  if (bc) HIP_TRY(c, hipMemcpyAsync(dbc, bc, d1, hipMemcpyHostToDevice, c->stream));
+  PropParams p{};
+  p.y0 = dy0;
+  p.bc = bc ? dbc : nullptr;
+  p.t_eval = dte;
+  p.m_eval = (int)m;
+  p.t0c = t0;
+  p.ysamp = dout;
+  p.status = dst;
+  p.steps = dsp;
+  p.evals = dev;
+  p.rejects = drj;
+  const int rc2 = run_propagation(c, p, n, stats);
+  if (rc2 != BQP_OK && rc2 != BQP_E_ROWS_FAILED) return rc2;
+  HIP_TRY(c, hipMemcpy(out, dout, ds, hipMemcpyDeviceToHost));
+  if (row_status) HIP_TRY(c, hipMemcpy(row_status, dst, i1, hipMemcpyDeviceToHost));
+  return rc2;
+}
+
+int32_t bqp_get_tableau(int32_t method, bqp_tableau* out) {
+  static const bqp_tableau tabs[3] = {BQP_DOPRI5_TABLEAU, BQP_DOP853_TABLEAU, BQP_RK87_TABLEAU};
+  if (!out || method < 0 || method > 2) return BQP_E_INVALID;
+  *out = tabs[method];
+  return BQP_OK;
+}
+
+// Algorithmic FLOPs of one RHS evaluation (FMA = 2, mul/add/div/sqrt = 1),
+// counted from the column sweep above: setup 16, column 0 11 per entry,
+// columns j >= 1: 20 for the sectoral step + its contribution, 20 per
+// recursion entry; rotation 14; drag 30 when enabled.
+double bqp_flops_per_rhs(const bqp_ctx* c) {
+  if (!c || c->nmax < 0) return 0.0;
+  const double N = c->nmax;
+  double f = 16.0 + 11.0 * (N + 1.0);
+  for (int j = 1; j <= c->jmax; ++j) f += 20.0 + 20.0 * (N + 1.0 - j);
+  f += 14.0;
+  if (c->cfg.drag != BQP_DRAG_NONE) f += 30.0;
+  return f;
+}
+
+}  // extern "C"

@@ -1,0 +1,355 @@
+// Device-side physics for the batched propagator (gfx950, one trajectory per lane).
+//
+// Gravity: fully normalised Cunningham V/W recursion (Montenbruck & Gill,
+// "Satellite Orbits" Sec. 3.2.4-3.2.5, normalised per Holmes & Featherstone
+// 2002) evaluated as an m-outer column sweep.  Each lane keeps only the two
+// previous entries of the current column, so the per-lane state is O(1)
+// instead of the reference's dense (N+2)^2 V/W table
+// (SC/src/gravity.rs:758-933 fill, :1719-1856 summation).
+//
+// The summation is regrouped by V/W entry instead of by (C,S) term: entry
+// (n, j) contributes to the acceleration through the coefficients of the
+// three terms (n-1, j-1), (n-1, j), (n-1, j+1) that read it in the M-G sums
+// (eq. 3.33).  The host folds those coefficients, the normalisation factors
+// and GM/R^2 into 6 numbers per entry, so the lane consumes every V/W value
+// the moment it is produced:
+//
+//   entry (n, j) = { a, b, XV, XW, YV, YW, ZV, ZW }      (64 bytes)
+//   V(n,j) = a*z~*V(n-1,j) - b*rho~*V(n-2,j)   (same for W)
+//   ax += XV*V + XW*W;  ay += YV*V + YW*W;  az += ZV*V + ZW*W
+//
+// For the column's first (sectoral) entry, a holds the sectoral factor.
+// All entries are wave-uniform, so the table is read through scalar loads
+// (SGPR operands), not per-lane memory traffic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tableaus.h"
+
+namespace bqp {
+
+struct GravEntry {
+  double a, b, xv, xw, yv, yw, zv, zw;
+};
+
+struct GravEntryF {
+  float a, b, xv, xw, yv, yw, zv, zw;
+};
+
+// Offset of column j in the entry table of a degree-N field.  Column 0 holds
+// n = 1..N+1 (N+1 entries); column j >= 1 holds n = j..N+1 (N+2-j entries).
+__host__ __device__ inline int col_offset(int N, int j) {
+  return j == 0 ? 0 : (N + 1) + (j - 1) * (N + 2) - ((j - 1) * j) / 2;
+}
+
+// Degree split of the split-precision form: entries with n <= SPLIT_N (field
+// degree n-1 <= 4: central term and J2..J4) stay FP64, the rest run in FP32.
+constexpr int SPLIT_N = 5;
+
+struct FieldView {
+  const GravEntry* tab;    // FP64 entries, all columns
+  const GravEntryF* tabf;  // FP32 copy of the same entries (split precision)
+  int nmax;                // field degree N
+  int jmax;                // last column = min(N+1, M+1)
+  double radius;
+};
+
+// Owner wave of column j when G waves share a trajectory group: columns are
+// dealt in a back-and-forth ("snake") order so long and short columns mix.
+template <int G>
+__device__ __forceinline__ int col_owner(int j) {
+  if (G == 1) return 0;
+  const int k = j / G, r = j - k * G;
+  return (k & 1) ? (G - 1 - r) : r;
+}
+
+// FP64 gravity, body-fixed position (m) -> acceleration (m/s^2), for the
+// columns owned by wave g of G.  The caller sums the G partial results.
+template <int G>
+__device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, double y, double z,
+                                          double& ax, double& ay, double& az) {
+  const double R = f.radius;
+  const double r2 = fma(x, x, fma(y, y, z * z));
+  const double rr = 1.0 / r2;
+  const double xt = x * R * rr, yt = y * R * rr, zt = z * R * rr, rho = R * R * rr;
+  double vs = R / sqrt(r2), ws = 0.0;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  const int top = f.nmax + 1;
+  const GravEntry* __restrict__ e = f.tab;
+  if (col_owner<G>(0) == g) {
+    double v1 = vs, v2 = 0.0;
+    int n = 1;
+    // two entries per trip: the pair's results land in (v2, v1) without moves
+    for (; n + 1 <= top; n += 2, e += 2) {
+      const GravEntry e0 = e[0], e1 = e[1];
+      const double A0 = e0.a * zt, B0 = e0.b * rho;
+      v2 = fma(A0, v1, -(B0 * v2));
+      sx = fma(e0.xv, v2, sx);
+      sy = fma(e0.yv, v2, sy);
+      sz = fma(e0.zv, v2, sz);
+      const double A1 = e1.a * zt, B1 = e1.b * rho;
+      v1 = fma(A1, v2, -(B1 * v1));
+      sx = fma(e1.xv, v1, sx);
+      sy = fma(e1.yv, v1, sy);
+      sz = fma(e1.zv, v1, sz);
+    }
+    if (n <= top) {
+      const double A = e->a * zt, B = e->b * rho;
+      const double v = fma(A, v1, -(B * v2));
+      sx = fma(e->xv, v, sx);
+      sy = fma(e->yv, v, sy);
+      sz = fma(e->zv, v, sz);
+      ++e;
+    }
+  } else {
+    e += top;
+  }
+  for (int j = 1; j <= f.jmax; ++j) {
+    const double s = e->a;
+    const double nv = s * fma(xt, vs, -(yt * ws));
+    const double nw = s * fma(xt, ws, yt * vs);
+    vs = nv;
+    ws = nw;
+    const int len = top + 1 - j;
+    if (col_owner<G>(j) == g) {
+      sx = fma(e->xv, vs, sx);
+      sx = fma(e->xw, ws, sx);
+      sy = fma(e->yv, vs, sy);
+      sy = fma(e->yw, ws, sy);
+      sz = fma(e->zv, vs, sz);
+      sz = fma(e->zw, ws, sz);
+      ++e;
+      double v1 = vs, w1 = ws, v2 = 0.0, w2 = 0.0;
+      int k = 1;
+      for (; k + 1 < len; k += 2, e += 2) {
+        const GravEntry e0 = e[0], e1 = e[1];
+        const double A0 = e0.a * zt, B0 = e0.b * rho;
+        v2 = fma(A0, v1, -(B0 * v2));
+        w2 = fma(A0, w1, -(B0 * w2));
+        sx = fma(e0.xv, v2, sx);
+        sx = fma(e0.xw, w2, sx);
+        sy = fma(e0.yv, v2, sy);
+        sy = fma(e0.yw, w2, sy);
+        sz = fma(e0.zv, v2, sz);
+        sz = fma(e0.zw, w2, sz);
+        const double A1 = e1.a * zt, B1 = e1.b * rho;
+        v1 = fma(A1, v2, -(B1 * v1));
+        w1 = fma(A1, w2, -(B1 * w1));
+        sx = fma(e1.xv, v1, sx);
+        sx = fma(e1.xw, w1, sx);
+        sy = fma(e1.yv, v1, sy);
+        sy = fma(e1.yw, w1, sy);
+        sz = fma(e1.zv, v1, sz);
+        sz = fma(e1.zw, w1, sz);
+      }
+      if (k < len) {
+        const double A = e->a * zt, B = e->b * rho;
+        const double v = fma(A, v1, -(B * v2));
+        const double w = fma(A, w1, -(B * w2));
+        sx = fma(e->xv, v, sx);
+        sx = fma(e->xw, w, sx);
+        sy = fma(e->yv, v, sy);
+        sy = fma(e->yw, w, sy);
+        sz = fma(e->zv, v, sz);
+        sz = fma(e->zw, w, sz);
+        ++e;
+      }
+    } else {
+      e += len;
+    }
+  }
+  ax = sx;
+  ay = sy;
+  az = sz;
+}
+
+// Split-precision gravity: entries with n <= SPLIT_N (degree <= 4) in FP64,
+// the rest in FP32.  Each FP32 column sum is short (<= N+1 terms) and is
+// folded into an FP32 Neumaier-compensated accumulator, whose value and
+// compensation are added to the FP64 sum at the end.
+struct Comp32 {
+  float s, c;
+};
+__device__ __forceinline__ void comp_add(Comp32& acc, float v) {
+  const float t = acc.s + v;
+  const float big = fabsf(acc.s) >= fabsf(v) ? acc.s : v;
+  const float small = fabsf(acc.s) >= fabsf(v) ? v : acc.s;
+  acc.c += (big - t) + small;
+  acc.s = t;
+}
+
+template <int G>
+__device__ __forceinline__ void grav_split(const FieldView& f, int g, double x, double y, double z,
+                                           double& ax, double& ay, double& az) {
+  const double R = f.radius;
+  const double r2 = fma(x, x, fma(y, y, z * z));
+  const double rr = 1.0 / r2;
+  const double xt = x * R * rr, yt = y * R * rr, zt = z * R * rr, rho = R * R * rr;
+  const float xtf = (float)xt, ytf = (float)yt, ztf = (float)zt, rhof = (float)rho;
+  double vs = R / sqrt(r2), ws = 0.0;
+  float vsf = 0.0f, wsf = 0.0f;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  Comp32 cx = {0.f, 0.f}, cy = {0.f, 0.f}, cz = {0.f, 0.f};
+  const int top = f.nmax + 1;
+  const GravEntry* __restrict__ e = f.tab;
+  const GravEntryF* __restrict__ ef = f.tabf;
+  for (int j = 0; j <= f.jmax; ++j) {
+    const int nlo = j == 0 ? 1 : j;
+    const int len = top + 1 - nlo;
+    if (j > 0) {
+      if (j <= SPLIT_N) {
+        const double s = e->a;
+        const double nv = s * fma(xt, vs, -(yt * ws));
+        const double nw = s * fma(xt, ws, yt * vs);
+        vs = nv;
+        ws = nw;
+        if (j == SPLIT_N) {
+          vsf = (float)vs;
+          wsf = (float)ws;
+        }
+      } else {
+        const float s = ef->a;
+        const float nv = s * fmaf(xtf, vsf, -(ytf * wsf));
+        const float nw = s * fmaf(xtf, wsf, ytf * vsf);
+        vsf = nv;
+        wsf = nw;
+      }
+    }
+    if (col_owner<G>(j) != g) {
+      e += len;
+      ef += len;
+      continue;
+    }
+    int n = nlo;
+    float v1f, w1f, v2f, w2f;
+    if (j <= SPLIT_N) {
+      // FP64 head of the column: n = nlo .. SPLIT_N
+      double v1, w1, v2 = 0.0, w2 = 0.0;
+      if (j == 0) {
+        v1 = vs;
+        w1 = 0.0;
+      } else {
+        v1 = vs;
+        w1 = ws;
+        sx = fma(e->xv, vs, sx);
+        sx = fma(e->xw, ws, sx);
+        sy = fma(e->yv, vs, sy);
+        sy = fma(e->yw, ws, sy);
+        sz = fma(e->zv, vs, sz);
+        sz = fma(e->zw, ws, sz);
+        ++e;
+        ++ef;
+        ++n;
+      }
+      for (; n <= SPLIT_N && n <= top; ++n, ++e, ++ef) {
+        const double A = e->a * zt, B = e->b * rho;
+        const double v = fma(A, v1, -(B * v2));
+        const double w = fma(A, w1, -(B * w2));
+        sx = fma(e->xv, v, sx);
+        sx = fma(e->xw, w, sx);
+        sy = fma(e->yv, v, sy);
+        sy = fma(e->yw, w, sy);
+        sz = fma(e->zv, v, sz);
+        sz = fma(e->zw, w, sz);
+        v2 = v1;
+        v1 = v;
+        w2 = w1;
+        w1 = w;
+      }
+      v1f = (float)v1;
+      w1f = (float)w1;
+      v2f = (float)v2;
+      w2f = (float)w2;
+    } else {
+      // FP32 sectoral entry
+      float px = ef->xv * vsf, py = ef->yv * vsf, pz = ef->zv * vsf;
+      px = fmaf(ef->xw, wsf, px);
+      py = fmaf(ef->yw, wsf, py);
+      pz = fmaf(ef->zw, wsf, pz);
+      comp_add(cx, px);
+      comp_add(cy, py);
+      comp_add(cz, pz);
+      ++e;
+      ++ef;
+      ++n;
+      v1f = vsf;
+      w1f = wsf;
+      v2f = 0.0f;
+      w2f = 0.0f;
+    }
+    // FP32 tail of the column
+    float px = 0.0f, py = 0.0f, pz = 0.0f;
+    for (; n <= top; ++n, ++e, ++ef) {
+      const float A = ef->a * ztf, B = ef->b * rhof;
+      const float v = fmaf(A, v1f, -(B * v2f));
+      const float w = fmaf(A, w1f, -(B * w2f));
+      px = fmaf(ef->xv, v, px);
+      px = fmaf(ef->xw, w, px);
+      py = fmaf(ef->yv, v, py);
+      py = fmaf(ef->yw, w, py);
+      pz = fmaf(ef->zv, v, pz);
+      pz = fmaf(ef->zw, w, pz);
+      v2f = v1f;
+      v1f = v;
+      w2f = w1f;
+      w1f = w;
+    }
+    comp_add(cx, px);
+    comp_add(cy, py);
+    comp_add(cz, pz);
+  }
+  ax = sx + ((double)cx.s + (double)cx.c);
+  ay = sy + ((double)cy.s + (double)cy.c);
+  az = sz + ((double)cz.s + (double)cz.c);
+}
+
+// ------------------------------------------------------------- atmosphere
+
+// Vallado (2013) Table 8-4: band base altitude (km), nominal density
+// (kg/m^3), scale height (km).
+__constant__ double c_vallado_h0[28] = {0,   25,  30,  40,  50,  60,  70,  80,  90,  100,
+                                        110, 120, 130, 140, 150, 180, 200, 250, 300, 350,
+                                        400, 450, 500, 600, 700, 800, 900, 1000};
+__constant__ double c_vallado_rho0[28] = {
+    1.225,     3.899e-2,  1.774e-2,  3.972e-3,  1.057e-3,  3.206e-4,  8.770e-5,
+    1.905e-5,  3.396e-6,  5.297e-7,  9.661e-8,  2.438e-8,  8.484e-9,  3.845e-9,
+    2.070e-9,  5.464e-10, 2.789e-10, 7.248e-11, 2.418e-11, 9.518e-12, 3.725e-12,
+    1.585e-12, 6.967e-13, 1.454e-13, 3.614e-14, 1.170e-14, 5.245e-15, 3.019e-15};
+__constant__ double c_vallado_H[28] = {7.249,  6.349,  6.682,  7.554,  8.382,  7.714,  6.549,
+                                       5.799,  5.382,  5.877,  7.263,  9.473,  12.636, 16.149,
+                                       22.523, 29.740, 37.105, 45.546, 53.628, 53.298, 58.515,
+                                       60.828, 63.822, 71.835, 88.667, 124.64, 181.05, 268.00};
+
+enum { DRAG_NONE = 0, DRAG_VALLADO = 1, DRAG_COMPAT = 2 };
+
+__device__ __forceinline__ double density(int model, double h_m) {
+  const double hk = h_m * 1e-3;
+  if (model == DRAG_VALLADO) {
+    // band search with wave-uniform table reads (scalar loads + selects)
+    double b0 = c_vallado_h0[0], r0 = c_vallado_rho0[0], H = c_vallado_H[0];
+#pragma unroll 4
+    for (int i = 1; i < 28; ++i) {
+      const bool up = hk >= c_vallado_h0[i];
+      b0 = up ? c_vallado_h0[i] : b0;
+      r0 = up ? c_vallado_rho0[i] : r0;
+      H = up ? c_vallado_H[i] : H;
+    }
+    if (hk < 0.0) return c_vallado_rho0[0];
+    return r0 * exp(-(hk - b0) / H);
+  }
+  if (model == DRAG_COMPAT) {
+    if (hk <= 0.0) return 1.225;
+    if (hk >= 1000.0) return 0.0;
+    return 1.225 * exp(-hk / 8.5);
+  }
+  return 0.0;
+}
+
+struct ModelView {
+  double omega, theta0, r_atm, r_min;
+  int drag;
+};
+
+}  // namespace bqp

@@ -1,0 +1,590 @@
+// Persistent batched propagation kernel (gfx950).
+//
+// One trajectory per lane.  A workgroup is G waves that share one group of
+// 64 trajectories: every wave carries the full integrator state of the 64
+// lanes, computes the gravity columns it owns (col_owner), and the G partial
+// accelerations are summed through LDS in a fixed order, so all G waves hold
+// bit-identical state and take identical (wave-uniform) control flow.
+//
+// Lanes pull rows from a device work queue (one atomic per wave per refill,
+// ranks by mbcnt), so trajectories of very different lengths share one launch;
+// a lane whose row finishes writes its result and takes the next row.
+//
+// The stepper restates the reference's explicit-RK engine:
+//   rk_step           LY/src/odesolve/solver.rs:1965-2173 (fma-axpy stage sums
+//                     with h*a_ij, skipped zero coefficients, Kahan-compensated
+//                     error sums)
+//   integrate_internal  solver.rs:1099-1925, PI controller macros :445-529
+//   compat preset     LY/src/odesolve/lightyear_compat.rs:100-600
+// Difference: after a rejected step k1 = f(t, y) is kept instead of being
+// re-evaluated (the value is bit-identical; only the eval count differs).
+#pragma once
+
+#include "bqp_device.hpp"
+
+#ifndef BQP_WAVES_PER_EU
+#define BQP_WAVES_PER_EU 1
+#endif
+
+namespace bqp {
+
+__constant__ bqp_tableau c_tab[3] = {BQP_DOPRI5_TABLEAU, BQP_DOP853_TABLEAU, BQP_RK87_TABLEAU};
+
+template <int METHOD>
+struct MethodTraits;
+template <>
+struct MethodTraits<BQP_METHOD_DOPRI5> {
+  static constexpr int S = 7;
+  static constexpr bool FSAL = true;
+  static constexpr bool E3 = false;
+};
+template <>
+struct MethodTraits<BQP_METHOD_DOP853> {
+  static constexpr int S = 12;
+  static constexpr bool FSAL = false;
+  static constexpr bool E3 = true;
+};
+template <>
+struct MethodTraits<BQP_METHOD_RK87> {
+  static constexpr int S = 13;
+  static constexpr bool FSAL = false;
+  static constexpr bool E3 = false;
+};
+
+enum { CONTROL_SCALED = 0, CONTROL_ABSOLUTE = 1, CONTROL_COMPAT = 2 };
+enum {
+  ROW_OK = 0,
+  ROW_MAX_STEPS = 1,
+  ROW_STEP_UNDERFLOW = 2,
+  ROW_NON_FINITE = 3,
+  ROW_MAX_REJECTS = 4,
+  ROW_INVALID_INPUT = 5,
+  ROW_BELOW_MIN_RADIUS = 6
+};
+
+struct PropParams {
+  const double* y0;
+  const double* t0;   // per row (final mode) or nullptr (sampled: t0c)
+  const double* tf;   // per row (final mode)
+  const double* bc;   // per row or nullptr
+  const int32_t* order;  // queue position -> row, or nullptr
+  const double* t_eval;  // sampled mode: m_eval sample times, else nullptr
+  double* yf;            // final mode (n x 6)
+  double* ysamp;         // sampled mode (n x m_eval x 6)
+  int32_t* status;
+  int32_t* steps;
+  int32_t* evals;
+  int32_t* rejects;
+  unsigned int* queue;
+  int n;
+  int m_eval;
+  double t0c;
+  FieldView field;
+  ModelView model;
+  int control;
+  int max_steps;
+  int max_rejects;
+  double rtol, atol_pos, atol_vel, eps, h0, h_min, h_max;
+};
+
+__device__ __forceinline__ double clampd(double v, double lo, double hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+__device__ __forceinline__ unsigned long long wave_ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int lane_rank(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+// Inertial Cowell RHS: uniform z-rotation to the body frame, gravity, rotate
+// back, exponential drag (LY/src/rhs.rs:1119-1163 compute_drag).
+template <int PREC, int G>
+__device__ __forceinline__ void rhs_eval(const PropParams& p, int g, int lane,
+                                         double (*lds)[3][64], double bc, double t,
+                                         const double* y, double* dy) {
+  const double th = fma(p.model.omega, t, p.model.theta0);
+  double s, c;
+  sincos(th, &s, &c);
+  const double xb = fma(c, y[0], s * y[1]);
+  const double yb = fma(c, y[1], -(s * y[0]));
+  const double zb = y[2];
+  double gx, gy, gz;
+  if (PREC == 0)
+    grav_fp64<G>(p.field, g, xb, yb, zb, gx, gy, gz);
+  else
+    grav_split<G>(p.field, g, xb, yb, zb, gx, gy, gz);
+  if (G > 1) {
+    lds[g][0][lane] = gx;
+    lds[g][1][lane] = gy;
+    lds[g][2][lane] = gz;
+    __syncthreads();
+    gx = lds[0][0][lane];
+    gy = lds[0][1][lane];
+    gz = lds[0][2][lane];
+#pragma unroll
+    for (int q = 1; q < G; ++q) {
+      gx += lds[q][0][lane];
+      gy += lds[q][1][lane];
+      gz += lds[q][2][lane];
+    }
+    __syncthreads();
+  }
+  double ax = fma(c, gx, -(s * gy));
+  double ay = fma(s, gx, c * gy);
+  double az = gz;
+  if (p.model.drag != DRAG_NONE && bc != 0.0) {
+    const double r = sqrt(fma(y[0], y[0], fma(y[1], y[1], y[2] * y[2])));
+    const double rho = density(p.model.drag, r - p.model.r_atm);
+    if (rho > 0.0) {
+      const double vrx = fma(p.model.omega, y[1], y[3]);
+      const double vry = fma(-p.model.omega, y[0], y[4]);
+      const double vrz = y[5];
+      const double vr = sqrt(fma(vrx, vrx, fma(vry, vry, vrz * vrz)));
+      const double k = -0.5 * bc * rho * vr;
+      ax = fma(k, vrx, ax);
+      ay = fma(k, vry, ay);
+      az = fma(k, vrz, az);
+    }
+  }
+  dy[0] = y[3];
+  dy[1] = y[4];
+  dy[2] = y[5];
+  dy[3] = ax;
+  dy[4] = ay;
+  dy[5] = az;
+}
+
+template <int METHOD, int PREC, int G>
+__global__ void __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(BQP_WAVES_PER_EU)))
+propagate_kernel(const PropParams p) {
+  using MT = MethodTraits<METHOD>;
+  constexpr int S = MT::S;
+  const bqp_tableau& T = c_tab[METHOD];
+  const int lane = threadIdx.x & 63;
+  const int g = threadIdx.x >> 6;
+  __shared__ double lds[G][3][64];
+  __shared__ unsigned int lds_base;
+
+  const bool compat = (p.control == CONTROL_COMPAT);
+  const bool sampled = (p.t_eval != nullptr);
+  const double q = compat ? 5.0 : (double)(T.order_err + 1);
+  const double inv_q = 1.0 / q;
+
+  bool active = false, exhausted = false;
+  int row = 0, steps = 0, evals = 0, rejects = 0, rej_run = 0, sidx = 0;
+  double t = 0.0, tseg = 0.0, tend = 0.0, h = 0.0, tcomp = 0.0, eprev = 0.0, bc = 0.0;
+  double hmin = 0.0, hmax = 0.0, eps = p.eps;
+  bool have_prev = false, just_rej = false, have_k1 = false;
+  double y[6] = {0, 0, 0, 0, 0, 0};
+  double k[S][6];
+#pragma unroll
+  for (int j = 0; j < S; ++j)
+#pragma unroll
+    for (int i = 0; i < 6; ++i) k[j][i] = 0.0;
+
+  // Writes the lane's row result and frees the lane.
+  auto finish = [&](int st) {
+    if (st != ROW_OK) {
+      if (sampled) {
+        for (int m = sidx; m < p.m_eval; ++m)
+          for (int i = 0; i < 6; ++i) p.ysamp[((size_t)row * p.m_eval + m) * 6 + i] = __builtin_nan("");
+      } else {
+        for (int i = 0; i < 6; ++i) p.yf[(size_t)row * 6 + i] = __builtin_nan("");
+      }
+    } else if (!sampled) {
+      for (int i = 0; i < 6; ++i) p.yf[(size_t)row * 6 + i] = y[i];
+    }
+    if (p.status) p.status[row] = st;
+    if (p.steps) p.steps[row] = steps;
+    if (p.evals) p.evals[row] = evals;
+    if (p.rejects) p.rejects[row] = rejects;
+    active = false;
+  };
+
+  while (true) {
+    // ---------------------------------------------------------- refill
+    const unsigned long long need = wave_ballot(!active);
+    if (need != 0ull && !exhausted) {
+      const int cnt = __popcll(need);
+      unsigned int base;
+      if (G == 1) {
+        unsigned int b = 0;
+        if (lane == __ffsll((long long)need) - 1) b = atomicAdd(p.queue, (unsigned)cnt);
+        base = __shfl(b, __ffsll((long long)need) - 1);
+      } else {
+        if (g == 0 && lane == __ffsll((long long)need) - 1) lds_base = atomicAdd(p.queue, (unsigned)cnt);
+        __syncthreads();
+        base = lds_base;
+        __syncthreads();
+      }
+      if (base + (unsigned)cnt >= (unsigned)p.n) exhausted = true;
+      if (!active) {
+        const unsigned pos = base + (unsigned)lane_rank(need);
+        if (pos < (unsigned)p.n) {
+          row = p.order ? p.order[pos] : (int)pos;
+          active = true;
+          steps = evals = rejects = rej_run = 0;
+          sidx = 0;
+          tcomp = 0.0;
+          eprev = 0.0;
+          have_prev = just_rej = have_k1 = false;
+          bc = p.bc ? p.bc[row] : 0.0;
+          bool ok = isfinite(bc);
+          for (int i = 0; i < 6; ++i) {
+            y[i] = p.y0[(size_t)row * 6 + i];
+            ok = ok && isfinite(y[i]);
+          }
+          t = sampled ? p.t0c : p.t0[row];
+          tend = sampled ? p.t_eval[p.m_eval - 1] : p.tf[row];
+          ok = ok && isfinite(t) && isfinite(tend) && (tend >= t);
+          if (!ok) {
+            finish(ROW_INVALID_INPUT);
+          } else if (p.model.r_min > 0.0 &&
+                     !(sqrt(fma(y[0], y[0], fma(y[1], y[1], y[2] * y[2]))) >= p.model.r_min)) {
+            finish(ROW_BELOW_MIN_RADIUS);
+          } else {
+            // samples that coincide with the start are written immediately
+            if (sampled) {
+              while (sidx < p.m_eval && p.t_eval[sidx] == t) {
+                for (int i = 0; i < 6; ++i) p.ysamp[((size_t)row * p.m_eval + sidx) * 6 + i] = y[i];
+                ++sidx;
+              }
+            }
+            const double span = tend - t;
+            if (span == 0.0 || (sampled && sidx >= p.m_eval)) {
+              finish(ROW_OK);
+            } else {
+              tseg = sampled ? p.t_eval[sidx] : tend;
+              if (compat) {
+                eps = fmax(p.eps, 1e-12);
+                hmin = 1e-10;
+                hmax = (p.h_max > 0.0 && isfinite(p.h_max)) ? fmax(p.h_max, hmin) : span;
+                h = clampd(fabs(span / 100.0), 1e-3, fmax(hmax, 1e-3));
+              } else {
+                hmax = (p.h_max > 0.0 && isfinite(p.h_max)) ? p.h_max : span;
+                hmin = (p.h_min > 0.0) ? p.h_min : 1e-12;
+                if (hmax < hmin) hmax = hmin;
+                const double h0 = (p.h0 > 0.0) ? p.h0 : ((span <= 60.0) ? span / 2.0 : span / 100.0);
+                h = clampd(h0, hmin, hmax);
+              }
+            }
+          }
+        }
+      }
+    }
+    if (wave_ballot(active) == 0ull) {
+      if (exhausted) break;
+      continue;
+    }
+
+    // ------------------------------------------------- one step attempt
+    bool stepping = active;
+    const double rem = tseg - t;
+    const double hs = (h < rem) ? h : rem;
+    const bool lands = (hs == rem);
+    if (stepping) {
+      const int budget = compat ? steps + rejects : steps;
+      if (budget >= p.max_steps || steps + rejects >= 2 * p.max_steps) {
+        finish(ROW_MAX_STEPS);
+        stepping = false;
+      } else if (!compat && hs < hmin && !lands) {
+        finish(ROW_STEP_UNDERFLOW);
+        stepping = false;
+      }
+    }
+    // stage 1 (k1): kept across rejects, reused after FSAL accepts
+    {
+      const bool needk = stepping && !have_k1;
+      if (wave_ballot(needk) != 0ull) {
+        double kk[6];
+        rhs_eval<PREC, G>(p, g, lane, lds, bc, t, y, kk);
+        if (needk) {
+#pragma unroll
+          for (int i = 0; i < 6; ++i) k[0][i] = kk[i];
+          ++evals;
+          have_k1 = true;
+        }
+      }
+    }
+    if (wave_ballot(stepping) == 0ull) continue;
+#pragma clang loop unroll(disable)
+    for (int s = 1; s < S; ++s) {
+      double ys[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) ys[i] = y[i];
+#pragma unroll
+      for (int j = 0; j < S - 1; ++j) {
+        if (j < s) {
+          const double a = T.a[s][j];
+          if (a != 0.0) {
+            const double sc = hs * a;
+#pragma unroll
+            for (int i = 0; i < 6; ++i) ys[i] = fma(k[j][i], sc, ys[i]);
+          }
+        }
+      }
+      double kk[6];
+      rhs_eval<PREC, G>(p, g, lane, lds, bc, t + T.c[s] * hs, ys, kk);
+#pragma unroll
+      for (int j = 1; j < S; ++j)
+        if (j == s) {
+#pragma unroll
+          for (int i = 0; i < 6; ++i) k[j][i] = kk[i];
+        }
+      if (stepping) ++evals;
+    }
+    if (!stepping) continue;
+
+    // solution and error estimates
+    double yn[6], err[6], comp[6], err3[6], comp3[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      yn[i] = y[i];
+      err[i] = comp[i] = err3[i] = comp3[i] = 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const double bj = T.b[j], ej = T.e[j];
+      if (bj != 0.0) {
+        const double hb = hs * bj;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) yn[i] = fma(k[j][i], hb, yn[i]);
+      }
+      if (ej != 0.0) {
+        const double he = hs * ej;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          const double term = he * k[j][i];
+          const double yk = term - comp[i];
+          const double acc = err[i] + yk;
+          comp[i] = (acc - err[i]) - yk;
+          err[i] = acc;
+        }
+      }
+      if (MT::E3) {
+        const double e3 = T.e3[j];
+        if (e3 != 0.0) {
+          const double he3 = hs * e3;
+#pragma unroll
+          for (int i = 0; i < 6; ++i) {
+            const double term = he3 * k[j][i];
+            const double yk = term - comp3[i];
+            const double acc = err3[i] + yk;
+            comp3[i] = (acc - err3[i]) - yk;
+            err3[i] = acc;
+          }
+        }
+      }
+    }
+    bool finite = true;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) finite = finite && isfinite(yn[i]) && isfinite(err[i]);
+    double en = 0.0, thresh = 1.0;
+    if (finite) {
+      const bool blend3 = MT::E3 && (p.control == CONTROL_SCALED ||
+                                     (p.control == CONTROL_ABSOLUTE && eps >= 1e-6));
+      if (p.control == CONTROL_SCALED) {
+        double s5 = 0.0, s3 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          const double atol = (i < 3) ? p.atol_pos : p.atol_vel;
+          const double sc = atol + p.rtol * fmax(fabs(y[i]), fabs(yn[i]));
+          const double den = (sc > 0.0) ? sc : atol;
+          const double e5 = err[i] / den;
+          s5 = fma(e5, e5, s5);
+          if (blend3) {
+            const double e3 = err3[i] / den;
+            s3 = fma(e3, e3, s3);
+          }
+        }
+        const double rms5 = sqrt(s5 / 6.0);
+        if (blend3) {
+          const double rms3 = sqrt(s3 / 6.0);
+          en = sqrt(fma(rms5, rms5, 0.01 * rms3 * rms3));
+        } else {
+          en = rms5;
+        }
+      } else {
+        double m5 = 0.0, m3 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          m5 = fmax(m5, fabs(err[i]));
+          if (blend3 && !compat) m3 = fmax(m3, fabs(err3[i]));
+        }
+        en = (blend3 && !compat) ? sqrt(fma(m5, m5, 0.01 * m3 * m3)) : m5;
+        thresh = eps;
+      }
+      finite = isfinite(en);
+    }
+    if (!finite) {
+      ++rejects;
+      if (++rej_run > p.max_rejects) {
+        finish(ROW_NON_FINITE);
+      } else {
+        h = clampd(hs * 0.1, hmin, hmax);
+      }
+      continue;
+    }
+    if (en <= thresh || hs <= hmin) {
+      double tn;
+      if (lands) {
+        tn = tseg;
+      } else {
+        const double ky = hs - tcomp;
+        tn = t + ky;
+        tcomp = (tn - t) - ky;
+      }
+      ++steps;
+      rej_run = 0;
+      if (MT::FSAL) {
+#pragma unroll
+        for (int i = 0; i < 6; ++i) k[0][i] = k[S - 1][i];
+        have_k1 = true;
+      } else {
+        have_k1 = false;
+      }
+      t = tn;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) y[i] = yn[i];
+      const double eff = (en > 0.0) ? en : thresh * 0.1;
+      double fac, maxg;
+      if (compat) {
+        fac = (have_prev && eprev > 0.0) ? 0.9 * pow(eps / eff, 0.7 / 5.0) * pow(eprev / eps, 0.4 / 5.0)
+                                         : 0.9 * pow(eps / eff, 0.2);
+        maxg = just_rej ? 2.0 : 5.0;
+      } else {
+        if (p.control == CONTROL_SCALED)
+          fac = (have_prev && eprev > 0.0) ? 0.9 * pow(1.0 / eff, 0.7 / q) * pow(eprev, 0.4 / q)
+                                           : 0.9 * pow(eff, -inv_q);
+        else
+          fac = (have_prev && eprev > 0.0)
+                    ? 0.9 * pow(eps / eff, 0.7 / q) * pow(eprev / eps, 0.4 / q)
+                    : 0.9 * pow(eps / eff, inv_q);
+        const double base = fmin(1.0 + 4.0 * (5.0 / q), 5.0);
+        maxg = just_rej ? fmin(base, 2.0) : base;
+      }
+      fac = clampd(fac, 0.2, maxg);
+      h = clampd(hs * fac, hmin, hmax);
+      eprev = en;
+      have_prev = true;
+      just_rej = false;
+      if (p.model.r_min > 0.0 &&
+          !(sqrt(fma(y[0], y[0], fma(y[1], y[1], y[2] * y[2]))) >= p.model.r_min)) {
+        finish(ROW_BELOW_MIN_RADIUS);
+      } else if (t == tseg) {
+        if (sampled) {
+          for (int i = 0; i < 6; ++i) p.ysamp[((size_t)row * p.m_eval + sidx) * 6 + i] = y[i];
+          ++sidx;
+          if (sidx >= p.m_eval) {
+            finish(ROW_OK);
+          } else {
+            tseg = p.t_eval[sidx];
+            tcomp = 0.0;
+          }
+        } else {
+          finish(ROW_OK);
+        }
+      }
+    } else {
+      ++rejects;
+      if (++rej_run > p.max_rejects) {
+        finish(ROW_MAX_REJECTS);
+      } else {
+        double fac;
+        if (p.control == CONTROL_SCALED)
+          fac = 0.9 * pow(en, -inv_q);
+        else
+          fac = 0.9 * pow(eps / en, compat ? 0.2 : inv_q);
+        fac = clampd(fac, 0.1, 0.5);
+        h = clampd(hs * fac, hmin, hmax);
+        if (compat) {
+          eprev = en;
+          have_prev = true;
+        }
+        just_rej = true;
+      }
+    }
+  }
+}
+
+// Standalone batched gravity: one body-fixed position per lane.
+template <int PREC>
+__global__ void __launch_bounds__(256) gravity_kernel(FieldView f, const double* __restrict__ pos,
+                                                      double* __restrict__ acc, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = i < n;
+  const long long r = ok ? i : 0;
+  const double x = pos[3 * r], y = pos[3 * r + 1], z = pos[3 * r + 2];
+  double ax, ay, az;
+  if (PREC == 0)
+    grav_fp64<1>(f, 0, x, y, z, ax, ay, az);
+  else
+    grav_split<1>(f, 0, x, y, z, ax, ay, az);
+  if (ok) {
+    acc[3 * i] = ax;
+    acc[3 * i + 1] = ay;
+    acc[3 * i + 2] = az;
+  }
+}
+
+// Queue reset without hipMemsetAsync (graph-capture safe).
+__global__ void reset_counter_kernel(unsigned int* q) { q[0] = 0u; }
+
+// Cost key per row for longest-first hand-out: orbit revolutions over the
+// span (span * mean motion from vis-viva), so rows of similar length share
+// a wave.
+__global__ void cost_key_kernel(const double* __restrict__ y0, const double* __restrict__ t0,
+                                const double* __restrict__ tf, double mu, int n,
+                                float* __restrict__ key, int32_t* __restrict__ idx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* s = y0 + 6 * (size_t)i;
+  const double r = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+  const double v2 = s[3] * s[3] + s[4] * s[4] + s[5] * s[5];
+  const double en = 0.5 * v2 - mu / r;
+  double rev = 0.0;
+  if (en < 0.0 && r > 0.0) {
+    const double a = -mu / (2.0 * en);
+    rev = (tf[i] - t0[i]) * sqrt(mu / (a * a * a));
+  }
+  key[i] = isfinite(rev) ? (float)rev : 0.0f;
+  idx[i] = i;
+}
+
+// Sum of per-row counters into 64-bit totals (stats).
+__global__ void row_stats_kernel(const int32_t* __restrict__ status, const int32_t* __restrict__ steps,
+                                 const int32_t* __restrict__ evals, const int32_t* __restrict__ rejects,
+                                 int n, unsigned long long* __restrict__ out) {
+  __shared__ unsigned long long acc[5];
+  if (threadIdx.x < 5) acc[threadIdx.x] = 0ull;
+  __syncthreads();
+  unsigned long long s = 0, e = 0, r = 0, ok = 0, bad = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    s += (unsigned)steps[i];
+    e += (unsigned)evals[i];
+    r += (unsigned)rejects[i];
+    if (status[i] == 0) ++ok; else ++bad;
+  }
+  atomicAdd(&acc[0], s);
+  atomicAdd(&acc[1], e);
+  atomicAdd(&acc[2], r);
+  atomicAdd(&acc[3], ok);
+  atomicAdd(&acc[4], bad);
+  __syncthreads();
+  if (threadIdx.x < 5) atomicAdd(&out[threadIdx.x], acc[threadIdx.x]);
+}
+
+__global__ void first_failed_kernel(const int32_t* __restrict__ status, int n, int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && status[i] != 0) atomicMin(out, i);
+}
+
+__global__ void fill_int_kernel(int* p, int v) { p[0] = v; }
+__global__ void zero_u64_kernel(unsigned long long* p, int n) {
+  const int i = threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+}  // namespace bqp

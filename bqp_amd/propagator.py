@@ -1,0 +1,216 @@
+"""Host-side mirror of the reference's propagation interface over the HIP C ABI.
+
+``Propagator`` owns one bqp context (one HIP stream, one device-resident
+field) and exposes the batch call shapes:
+
+* ``propagate_batch_final``   ragged rows (y0, t0, tf, Cd*A/m) -> final states,
+  the shape of ``LightyearSession::integrate_variable_final_into``
+  (dust_transfer/crates/lightyear_odeint_rs/src/session.rs:2325-2436);
+* ``propagate_batch_sampled`` fixed sample grid -> (n, m, 6), the shape of
+  ``integrate_batch_native_into`` (.../src/batch.rs:247-390) in inertial states;
+* ``propagate_batch_final_dev`` the same on device-resident torch tensors;
+* ``gravity_accel``          body-fixed accelerations
+  (crates/satpy_core/src/gravity.rs:2638 ``spherical_gravity_impl_frame_packed``).
+
+Errors follow the reference: bad arguments raise ``ValueError``; a device or
+integration failure raises ``RuntimeError`` (``BqpError``); batch rows that fail
+are NaN-filled and reported in the returned status array.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as nat
+from .elements import MU_EARTH, OMEGA_EARTH, R_EARTH
+from .field import Field
+from .tableau_struct import Tableau, to_dict
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def tableau(method: str) -> dict:
+    t = Tableau()
+    rc = nat.load().bqp_get_tableau(nat.METHODS[method], C.byref(t))
+    if rc != 0:
+        raise ValueError(method)
+    return to_dict(t)
+
+
+class Propagator:
+    def __init__(self, field: Field, *, method: str = "rk87", control: str = "scaled",
+                 precision: str = "fp64", drag: str = "none", rtol: float = 1e-12,
+                 atol_pos: float | None = None, atol_vel: float | None = None, eps: float = 1e-8,
+                 h0: float = 0.0, h_min: float = 0.0, h_max: float = 0.0, max_steps: int = 500000,
+                 max_rejects: int = 50, omega: float = OMEGA_EARTH, theta0: float = 0.0,
+                 r_atm: float = R_EARTH, r_min: float = 0.0, launch: dict | None = None):
+        self._lib = nat.load()
+        self._ctx = C.c_void_p()
+        self.field = field
+        cfg = self._make_config(method, control, precision, drag, rtol, atol_pos, atol_vel, eps, h0, h_min,
+                                h_max, max_steps, max_rejects, omega, theta0, r_atm, r_min, field)
+        rc = self._lib.bqp_create(C.byref(cfg), C.byref(self._ctx))
+        if rc != 0:
+            if rc == nat.E_INVALID:
+                raise ValueError("invalid propagator configuration")
+            raise nat.BqpError(f"bqp_create failed ({rc}): no usable HIP device?")
+        self.config = cfg
+        self._check(self._lib.bqp_load_field(self._ctx, field.nmax, field.mmax, _dp(field.cbar), _dp(field.sbar),
+                                             field.mu, field.radius))
+        if launch:
+            self.set_launch(**launch)
+
+    # ------------------------------------------------------------ plumbing
+    @staticmethod
+    def _make_config(method, control, precision, drag, rtol, atol_pos, atol_vel, eps, h0, h_min, h_max,
+                     max_steps, max_rejects, omega, theta0, r_atm, r_min, field) -> nat.Config:
+        for name, table, val in (("method", nat.METHODS, method), ("control", nat.CONTROLS, control),
+                                 ("precision", nat.PRECISIONS, precision), ("drag", nat.DRAGS, drag)):
+            if val not in table:
+                raise ValueError(f"unknown {name} {val!r}; use one of {sorted(table)}")
+        du = field.radius if field is not None else R_EARTH
+        mu = field.mu if field is not None else MU_EARTH
+        vu = float(np.sqrt(mu / du))
+        cfg = nat.Config()
+        cfg.method = nat.METHODS[method]
+        cfg.control = nat.CONTROLS[control]
+        cfg.precision = nat.PRECISIONS[precision]
+        cfg.drag = nat.DRAGS[drag]
+        cfg.rtol = rtol
+        cfg.atol_pos = rtol * du if atol_pos is None else atol_pos
+        cfg.atol_vel = rtol * vu if atol_vel is None else atol_vel
+        cfg.eps = eps
+        cfg.h0, cfg.h_min, cfg.h_max = h0, h_min, h_max
+        cfg.max_steps, cfg.max_rejects = int(max_steps), int(max_rejects)
+        cfg.omega, cfg.theta0, cfg.r_atm, cfg.r_min = omega, theta0, r_atm, r_min
+        return cfg
+
+    def _check(self, rc: int, allow_rows: bool = False):
+        if rc == 0 or (allow_rows and rc == nat.E_ROWS_FAILED):
+            return
+        msg = self._lib.bqp_last_error(self._ctx).decode()
+        if rc == nat.E_INVALID:
+            raise ValueError(msg)
+        raise nat.BqpError(f"bqp error {rc}: {msg}")
+
+    def close(self):
+        if getattr(self, "_ctx", None) is not None and self._ctx.value:
+            self._lib.bqp_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_launch(self, waves_per_group: int = 0, groups_per_block: int = 0, blocks: int = 0,
+                   sort_by_cost: int = 1):
+        L = nat.Launch(waves_per_group, groups_per_block, blocks, sort_by_cost)
+        self._check(self._lib.bqp_set_launch(self._ctx, C.byref(L)))
+
+    def reconfigure(self, **kw):
+        """Change integrator/model settings (same keywords as the constructor)."""
+        cur = {k: getattr(self.config, k) for k, _ in nat.Config._fields_}
+        names = {"method": nat.METHODS, "control": nat.CONTROLS, "precision": nat.PRECISIONS, "drag": nat.DRAGS}
+        for k, v in kw.items():
+            if k in names:
+                if v not in names[k]:
+                    raise ValueError(f"unknown {k} {v!r}")
+                cur[k] = names[k][v]
+            elif k in cur:
+                cur[k] = v
+            else:
+                raise ValueError(f"unknown setting {k!r}")
+        cfg = nat.Config(**cur)
+        self._check(self._lib.bqp_set_config(self._ctx, C.byref(cfg)))
+        self.config = cfg
+
+    @property
+    def flops_per_rhs(self) -> float:
+        return float(self._lib.bqp_flops_per_rhs(self._ctx))
+
+    # ------------------------------------------------------------ calls
+    def gravity_accel(self, pos_bf) -> np.ndarray:
+        pos = np.ascontiguousarray(pos_bf, dtype=np.float64).reshape(-1, 3)
+        out = np.empty_like(pos)
+        self._check(self._lib.bqp_gravity_accel_batch(self._ctx, _dp(pos), pos.shape[0], _dp(out)))
+        return out
+
+    def propagate_batch_final(self, y0, t0, tf, bc=None):
+        """Rows (n,6) SI from t0[i] to tf[i]; returns (yf, status, info)."""
+        y0 = np.ascontiguousarray(y0, dtype=np.float64).reshape(-1, 6)
+        n = y0.shape[0]
+        t0 = np.ascontiguousarray(np.broadcast_to(np.asarray(t0, dtype=np.float64), (n,)))
+        tf = np.ascontiguousarray(np.broadcast_to(np.asarray(tf, dtype=np.float64), (n,)))
+        bcp = None
+        if bc is not None:
+            bc = np.ascontiguousarray(np.broadcast_to(np.asarray(bc, dtype=np.float64), (n,)))
+            bcp = _dp(bc)
+        yf = np.empty_like(y0)
+        status = np.empty(n, dtype=np.int32)
+        steps = np.empty(n, dtype=np.int32)
+        evals = np.empty(n, dtype=np.int32)
+        st = nat.Stats()
+        rc = self._lib.bqp_propagate_batch_final(self._ctx, _dp(y0), _dp(t0), _dp(tf), bcp, n, _dp(yf),
+                                                 _ip(status), _ip(steps), _ip(evals), C.byref(st))
+        self._check(rc, allow_rows=True)
+        info = st.as_dict()
+        info["steps_per_row"] = steps
+        info["evals_per_row"] = evals
+        return yf, status, info
+
+    def propagate_batch_final_dev(self, y0, t0, tf, bc=None, yf=None, status=None, steps=None, evals=None):
+        """Device-resident variant on torch CUDA tensors (float64 / int32);
+        returns (yf, status, stats dict).  No host staging."""
+        import torch
+
+        n = y0.shape[0]
+        for name, t in (("y0", y0), ("t0", t0), ("tf", tf)):
+            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+                raise ValueError(f"{name} must be a contiguous float64 device tensor")
+        if yf is None:
+            yf = torch.empty_like(y0)
+        if status is None:
+            status = torch.empty(n, dtype=torch.int32, device=y0.device)
+        st = nat.Stats()
+        torch.cuda.synchronize(y0.device)
+        rc = self._lib.bqp_propagate_batch_final_dev(
+            self._ctx, C.c_void_p(y0.data_ptr()), C.c_void_p(t0.data_ptr()), C.c_void_p(tf.data_ptr()),
+            C.c_void_p(bc.data_ptr()) if bc is not None else None, n, C.c_void_p(yf.data_ptr()),
+            C.c_void_p(status.data_ptr()), C.c_void_p(steps.data_ptr()) if steps is not None else None,
+            C.c_void_p(evals.data_ptr()) if evals is not None else None, C.byref(st))
+        self._check(rc, allow_rows=True)
+        return yf, status, st.as_dict()
+
+    def propagate_batch_sampled(self, y0, t0: float, t_eval, bc=None):
+        """Rows (n,6) from a common t0, sampled at t_eval (m,); returns
+        (states (n, m, 6), status, info)."""
+        y0 = np.ascontiguousarray(y0, dtype=np.float64).reshape(-1, 6)
+        n = y0.shape[0]
+        te = np.ascontiguousarray(t_eval, dtype=np.float64).reshape(-1)
+        m = te.shape[0]
+        bcp = None
+        if bc is not None:
+            bc = np.ascontiguousarray(np.broadcast_to(np.asarray(bc, dtype=np.float64), (n,)))
+            bcp = _dp(bc)
+        out = np.empty((n, m, 6))
+        status = np.empty(n, dtype=np.int32)
+        st = nat.Stats()
+        rc = self._lib.bqp_propagate_batch_sampled(self._ctx, _dp(y0), float(t0), _dp(te), m, bcp, n, _dp(out),
+                                                   _ip(status), C.byref(st))
+        self._check(rc, allow_rows=True)
+        return out, status, st.as_dict()

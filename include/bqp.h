@@ -1,0 +1,161 @@
+/* bqp — MI355X-native batched orbit propagator: C ABI.
+ *
+ * Plain C types only (pointers, sizes, POD structs); no exceptions cross the
+ * boundary; every function returns an int32 status (0 = ok, see BQP_E_*).
+ * A context owns one HIP stream and the device copy of one gravity field; it
+ * is thread-compatible (one thread at a time), distinct contexts are
+ * independent.  Units are SI: m, m/s, s, m^2/kg.
+ *
+ * Reference interfaces replaced (tdude78/bqp, dust_transfer/):
+ *   bqp_load_field            crates/lightyear_odeint_rs/src/config.rs:65-223
+ *                             (load_constants / packed_constants_from_reader) and
+ *                             crates/satpy_core/src/gravity.rs:1570 pack_gravity_coeffs
+ *   bqp_gravity_accel_batch   crates/satpy_core/src/gravity.rs:2638
+ *                             spherical_gravity_impl_frame_packed (and the unlaunched
+ *                             CUDA mirror crates/satpy_core/kernels/cuda/gravity.cu:114)
+ *   bqp_propagate_batch_final crates/lightyear_odeint_rs/src/session.rs:2325
+ *                             LightyearSession::integrate_variable_final_into
+ *                             (VariableFinalBatchRequest, session.rs:2028-2035) and
+ *                             crates/lightyear_odeint_rs/src/batch.rs:247
+ *                             integrate_batch_native_into (final-only shape)
+ *   bqp_propagate_batch_sampled  crates/lightyear_odeint_rs/src/batch.rs:247
+ *                             integrate_batch_native_into with t_eval (n, m, 6)
+ *   bqp_config                crates/lightyear_odeint_rs/src/types.rs:332-520 ForceConfig
+ *                             (hot-path subset) + odesolve/solver.rs:37-62 IntegratorConfig
+ *   row status codes          odesolve/solver.rs:65-95 IntegrationStatus and
+ *                             session.rs:1822-1850 VariableFinalRowFailure
+ */
+#ifndef BQP_H
+#define BQP_H
+
+#include <stdint.h>
+
+#include "../bqp_amd/csrc/tableaus.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (function return values) */
+#define BQP_OK 0
+#define BQP_E_INVALID 1      /* bad argument / shape contract */
+#define BQP_E_NO_FIELD 2     /* no gravity field loaded (reference: CONSTANTS_NOT_LOADED) */
+#define BQP_E_DEVICE 3       /* HIP runtime error */
+#define BQP_E_ROWS_FAILED 4  /* call completed, at least one row failed (see row_status) */
+#define BQP_E_UNSUPPORTED 5
+
+/* ---- per-row status (mirrors IntegrationStatus / VariableFinalRowFailure) */
+#define BQP_ROW_OK 0
+#define BQP_ROW_MAX_STEPS 1
+#define BQP_ROW_STEP_UNDERFLOW 2
+#define BQP_ROW_NON_FINITE 3
+#define BQP_ROW_MAX_REJECTS 4
+#define BQP_ROW_INVALID_INPUT 5
+#define BQP_ROW_BELOW_MIN_RADIUS 6
+
+/* ---- enums */
+#define BQP_CONTROL_SCALED 0   /* WRMS, atol + rtol*max(|y|,|y'|) (solver.rs ErrorControl::Scaled) */
+#define BQP_CONTROL_ABSOLUTE 1 /* max-norm vs eps (ErrorControl::Absolute) */
+#define BQP_CONTROL_COMPAT 2   /* lightyear_compat.rs DOPRI5 controller preset */
+
+#define BQP_DRAG_NONE 0
+#define BQP_DRAG_VALLADO 1     /* Vallado 2013 Table 8-4 piecewise exponential */
+#define BQP_DRAG_COMPAT 2      /* reference AtmModel::Exponential (rhs.rs:1778-1788) */
+
+#define BQP_PRECISION_FP64 0   /* plain FP64 (the correctness oracle form) */
+#define BQP_PRECISION_SPLIT 1  /* degree <= 4 in FP64, higher degrees FP32 + compensated sums */
+
+typedef struct bqp_ctx bqp_ctx;
+
+typedef struct bqp_config {
+  int32_t method;      /* BQP_METHOD_DOPRI5 / _DOP853 / _RK87 (tableaus.h) */
+  int32_t control;     /* BQP_CONTROL_* */
+  int32_t precision;   /* BQP_PRECISION_* */
+  int32_t drag;        /* BQP_DRAG_* */
+  double rtol;         /* scaled control */
+  double atol_pos;     /* m */
+  double atol_vel;     /* m/s */
+  double eps;          /* absolute / compat control */
+  double h0;           /* <= 0: reference default (span/2 if span <= 60 s, else span/100) */
+  double h_min;        /* <= 0: 1e-12 s */
+  double h_max;        /* <= 0: unlimited (the row's span) */
+  int64_t max_steps;   /* <= 0: 500000 */
+  int32_t max_rejects; /* <= 0: 50 */
+  int32_t reserved0;
+  double omega;        /* body-frame rotation rate about +z (rad/s) */
+  double theta0;       /* body-frame angle at t = 0 (rad) */
+  double r_atm;        /* altitude = |r| - r_atm (m) */
+  double r_min;        /* rows whose |r| drops below r_min fail (0 = off) */
+} bqp_config;
+
+typedef struct bqp_stats {
+  int64_t steps;       /* accepted steps, summed over rows */
+  int64_t rejects;
+  int64_t evals;       /* RHS evaluations */
+  int64_t rows_ok;
+  int64_t rows_failed;
+  int64_t first_failed_row; /* -1 if none */
+  double kernel_ms;    /* device time of the propagation kernel (HIP events) */
+  double flops;        /* algorithmic FLOPs of the call */
+} bqp_stats;
+
+/* Launch shape (tuning); zero fields mean "default".  Chosen by the
+ * autotuner in bqp_amd.autotune when not given. */
+typedef struct bqp_launch {
+  int32_t waves_per_group;  /* G: waves sharing one 64-trajectory group (1, 2, 4) */
+  int32_t groups_per_block; /* trajectory groups per workgroup (1..4) */
+  int32_t blocks;           /* persistent grid size; 0 = fill the device */
+  int32_t sort_by_cost;     /* 1 = hand out rows in descending estimated cost */
+} bqp_launch;
+
+int32_t bqp_default_config(bqp_config* out);
+int32_t bqp_create(const bqp_config* cfg, bqp_ctx** out);
+int32_t bqp_destroy(bqp_ctx* ctx);
+int32_t bqp_set_config(bqp_ctx* ctx, const bqp_config* cfg);
+int32_t bqp_set_launch(bqp_ctx* ctx, const bqp_launch* launch);
+const char* bqp_last_error(const bqp_ctx* ctx);
+
+/* Normalised Cbar/Sbar, triangular row-major: idx(n, m) = n(n+1)/2 + m,
+ * n = 0..nmax.  Cbar[0] should be 1 (central term); mmax <= nmax. */
+int32_t bqp_load_field(bqp_ctx* ctx, int32_t nmax, int32_t mmax, const double* cbar,
+                       const double* sbar, double mu, double radius);
+
+/* Body-fixed gravity acceleration for n body-fixed positions (n x 3). */
+int32_t bqp_gravity_accel_batch(bqp_ctx* ctx, const double* pos_bf, int64_t n, double* acc);
+int32_t bqp_gravity_accel_batch_dev(bqp_ctx* ctx, const double* d_pos, int64_t n, double* d_acc);
+
+/* Ragged final-state batch.  Row i: inertial state y0[6i..6i+5] at t0[i],
+ * propagated to tf[i] >= t0[i] with ballistic coefficient bc[i] = Cd*A/m
+ * (NULL = no drag coefficient).  Failed rows are NaN-filled and flagged in
+ * row_status.  row_status / row_steps / row_evals may be NULL.  Returns
+ * BQP_E_ROWS_FAILED (after writing every row) if any row failed. */
+int32_t bqp_propagate_batch_final(bqp_ctx* ctx, const double* y0, const double* t0,
+                                  const double* tf, const double* bc, int64_t n, double* yf,
+                                  int32_t* row_status, int32_t* row_steps, int32_t* row_evals,
+                                  bqp_stats* stats);
+
+/* Same with device-resident buffers (no host staging; the caller's
+ * buffers must stay valid until the call returns).  Optional ones may be
+ * NULL. */
+int32_t bqp_propagate_batch_final_dev(bqp_ctx* ctx, const double* d_y0, const double* d_t0,
+                                      const double* d_tf, const double* d_bc, int64_t n,
+                                      double* d_yf, int32_t* d_row_status, int32_t* d_row_steps,
+                                      int32_t* d_row_evals, bqp_stats* stats);
+
+/* Fixed-grid sampled batch: every row starts at t0 and is sampled at the
+ * m increasing times t_eval[0..m-1] (>= t0), output (n, m, 6) inertial states.
+ * Samples are hit exactly (the step is shortened onto each sample time). */
+int32_t bqp_propagate_batch_sampled(bqp_ctx* ctx, const double* y0, double t0,
+                                    const double* t_eval, int64_t m, const double* bc, int64_t n,
+                                    double* out, int32_t* row_status, bqp_stats* stats);
+
+/* Butcher tableau of a method (for host-side checks). */
+int32_t bqp_get_tableau(int32_t method, bqp_tableau* out);
+
+/* Algorithmic FLOPs of one RHS evaluation at the loaded field / config. */
+double bqp_flops_per_rhs(const bqp_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BQP_H */

@@ -1,0 +1,122 @@
+/* CPU FP64 oracle for the batched orbit propagator.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against, and the CPU baseline that bench.py times.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product (bqp_amd/) never calls it and has no CPU fallback.
+ *
+ * It restates, in plain C, the reference algorithm of
+ *   dust_transfer/crates/satpy_core/src/gravity.rs   (V/W fill + M-G summation)
+ *   dust_transfer/crates/lightyear_odeint_rs/src/rhs.rs (density, drag, RHS)
+ *   dust_transfer/crates/lightyear_odeint_rs/src/odesolve/solver.rs (rk_step,
+ *       integrate_internal, PI controller)
+ *   dust_transfer/crates/lightyear_odeint_rs/src/odesolve/lightyear_compat.rs
+ *   dust_transfer/crates/lightyear_odeint_rs/src/session.rs (ragged batch rows)
+ * with the deviations the propagator's north star asks for: fully normalised
+ * coefficients, SI units, Cowell (not Encke) state, a uniform z-rotation for the
+ * body frame, and the Vallado piecewise exponential atmosphere.
+ *
+ * Pinning status of each function (what checks it):
+ *   orc_gravity_bf (normalised)   Orekit d/o-5 vectors, mpmath brute force
+ *                                  N = 8..120, point-mass and J2 closed forms,
+ *                                  Laplace / rotation invariance.
+ *   orc_gravity_bf (compat)       equality with the normalised form (1e-13).
+ *   orc_density                   closed form at the table nodes, continuity.
+ *   orc_tableau                   order conditions of every tableau.
+ *   orc_propagate                 two-body Kepler closed form, Jacobi
+ *                                  integral, tolerance convergence, controller
+ *                                  unit behaviours (PI memory, h_min, rejects).
+ *   orc_kep2eci                   round trip with an independent numpy
+ *                                  inverse and the vis-viva / angular-momentum
+ *                                  identities.
+ */
+#ifndef BQP_ORACLE_H
+#define BQP_ORACLE_H
+
+#include <stdint.h>
+#include "../bqp_amd/csrc/tableaus.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_NMAX 200
+
+/* Integration status codes (shared meaning with include/bqp.h). */
+enum {
+  ORC_OK = 0,
+  ORC_MAX_STEPS = 1,
+  ORC_STEP_UNDERFLOW = 2,
+  ORC_NON_FINITE = 3,
+  ORC_MAX_REJECTS = 4,
+  ORC_INVALID_INPUT = 5,
+  ORC_BELOW_MIN_RADIUS = 6
+};
+
+enum { ORC_CONTROL_SCALED = 0, ORC_CONTROL_ABSOLUTE = 1, ORC_CONTROL_COMPAT = 2 };
+enum { ORC_DRAG_NONE = 0, ORC_DRAG_VALLADO = 1, ORC_DRAG_COMPAT = 2 };
+enum { ORC_GRAVITY_NORMALISED = 0, ORC_GRAVITY_COMPAT = 1 };
+
+/* Prepared field: normalised Cbar/Sbar (triangular, idx = n(n+1)/2 + m) plus
+ * the recursion and summation constants derived from them. */
+typedef struct orc_field orc_field;
+
+orc_field* orc_field_create(int nmax, int mmax, const double* cbar, const double* sbar,
+                            double mu, double radius, int form);
+void orc_field_destroy(orc_field* f);
+
+/* Body-fixed acceleration (m/s^2) of a body-fixed position (m). */
+int orc_gravity_bf(const orc_field* f, const double pos[3], double acc[3]);
+int orc_gravity_bf_batch(const orc_field* f, const double* pos, int64_t n, double* acc,
+                         int threads);
+
+double orc_density(int model, double h_m);
+
+typedef struct orc_model {
+  const orc_field* field;
+  double omega;     /* body rotation rate about +z, rad/s */
+  double theta0;    /* body rotation angle at t = 0, rad */
+  int drag;         /* ORC_DRAG_* */
+  double r_atm;     /* radius subtracted from |r| to get altitude, m */
+  double r_min;     /* minimum admissible radius, m (0 disables) */
+} orc_model;
+
+typedef struct orc_integ {
+  int method;       /* bqp_method_id */
+  int control;      /* ORC_CONTROL_* */
+  double rtol;
+  double atol_pos, atol_vel;
+  double eps;       /* absolute / compat tolerance */
+  double h0;        /* <= 0: reference default */
+  double h_min;     /* <= 0: 1e-12 */
+  double h_max;     /* <= 0: span */
+  int64_t max_steps;
+  int32_t max_rejects;
+} orc_integ;
+
+typedef struct orc_stats {
+  int64_t steps;    /* accepted steps (compat: attempted) */
+  int64_t rejects;
+  int64_t evals;
+  int32_t status;
+} orc_stats;
+
+void orc_rhs(const orc_model* m, double bc, double t, const double y[6], double dy[6]);
+
+int orc_propagate(const orc_model* m, const orc_integ* cfg, double bc, const double y0[6],
+                  double t0, double tf, double yf[6], orc_stats* st);
+
+/* Ragged batch: row i runs from t0[i] to tf[i] with ballistic coefficient
+ * bc[i] (Cd*A/m, m^2/kg; NULL = 0).  Failed rows are NaN-filled. */
+int orc_propagate_batch(const orc_model* m, const orc_integ* cfg, const double* y0,
+                        const double* t0, const double* tf, const double* bc, int64_t n,
+                        double* yf, int32_t* status, int64_t* steps, int64_t* evals,
+                        int threads);
+
+void orc_kep2eci(double mu, const double kep[6], double eci[6]);
+int orc_tableau(int method, bqp_tableau* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,171 @@
+"""GPU propagation against the CPU oracle on the same seeded inputs.
+
+Bars: final positions within 1 cm of the oracle at equal tolerances (the
+north-star parity bar; observed differences are far smaller), bit-identical
+results across launch shapes, NaN-filled failed rows with their status."""
+import numpy as np
+import pytest
+
+import bqp_amd
+from bqp_amd import inputs
+from gpu_util import announce, oracle_pair, pos_err
+
+pytestmark = pytest.mark.gpu
+
+DAY = 86400.0
+
+
+def _run_both(orc, field, w, n, method, rtol, drag, tf=None, control="scaled", eps=1e-8, launch=None,
+              h_max=0.0):
+    y0 = w["y0"][:n]
+    t0 = np.zeros(n)
+    tf = w["tf"][:n] if tf is None else np.full(n, tf)
+    bc = w["bc"][:n] if drag != "none" else None
+    with bqp_amd.Propagator(field, method=method, rtol=rtol, drag=drag, control=control, eps=eps,
+                            launch=launch, h_max=h_max) as p:
+        yg, sg, info = p.propagate_batch_final(y0, t0, tf, bc)
+    _, model, integ = oracle_pair(orc, field, method=method, rtol=rtol, drag=drag, control=control, eps=eps,
+                                  h_max=h_max)
+    yo, so, steps, evals = orc.propagate_batch(model, integ, y0, t0, tf, bc)
+    return yg, sg, info, yo, so, steps
+
+
+def test_w0_dopri5_vs_oracle(orc):
+    announce("w0 DOPRI5 N=8 1 day, 256 rows vs oracle")
+    w = inputs.workload("w0", n=256)
+    field = bqp_amd.kaula(8)
+    yg, sg, info, yo, so, steps = _run_both(orc, field, w, 256, "dopri5", 1e-10, "none")
+    assert np.all(sg == 0) and np.all(so == 0)
+    err = pos_err(yg, yo)
+    assert err.max() < 1e-2, err.max()
+    # steps agree with the restated controller (rare ties may flip a decision)
+    assert np.mean(info["steps_per_row"] == steps) > 0.9
+
+
+def test_rk87_drag_vs_oracle(orc):
+    announce("RK8(7)13M N=20 Vallado drag 1 day, 64 rows vs oracle")
+    w = inputs.workload("w1", n=64)
+    field = bqp_amd.kaula(20)
+    yg, sg, info, yo, so, steps = _run_both(orc, field, w, 64, "rk87", 1e-12, "vallado", tf=DAY)
+    assert np.all(sg == 0) and np.all(so == 0)
+    assert pos_err(yg, yo).max() < 1e-2
+    assert np.max(np.abs(yg[:, 3:] - yo[:, 3:])) < 1e-5
+
+
+def test_dop853_vs_oracle(orc):
+    announce("DOP853 N=12 6 h, 64 rows vs oracle")
+    w = inputs.workload("w0", n=64)
+    field = bqp_amd.kaula(12)
+    yg, sg, info, yo, so, steps = _run_both(orc, field, w, 64, "dop853", 1e-11, "none", tf=6 * 3600.0)
+    assert np.all(sg == 0)
+    assert pos_err(yg, yo).max() < 1e-2
+
+
+def test_compat_controller_vs_oracle(orc):
+    announce("DOPRI5 compat controller (absolute eps) vs oracle")
+    w = inputs.workload("w0", n=64)
+    field = bqp_amd.kaula(5)
+    yg, sg, info, yo, so, steps = _run_both(orc, field, w, 64, "dopri5", 1e-12, "none", tf=3 * 3600.0,
+                                            control="compat", eps=1e-6, h_max=300.0)
+    assert np.all(sg == 0)
+    assert pos_err(yg, yo).max() < 1e-2
+
+
+def test_w1_seven_day_n70_vs_oracle(orc):
+    """The headline parity claim: 7-day LEO arcs at degree/order 70 with drag,
+    RK8(7)13M rtol 1e-12, final positions within 1 cm of the oracle."""
+    announce("w1 RK8(7)13M N=70 drag 7 days, 16 rows vs oracle")
+    w = inputs.workload("w1", n=16)
+    field = bqp_amd.kaula(70)
+    yg, sg, info, yo, so, steps = _run_both(orc, field, w, 16, "rk87", 1e-12, "vallado")
+    assert np.all(sg == 0) and np.all(so == 0)
+    err = pos_err(yg, yo)
+    announce(f"w1 parity max |dr| = {err.max():.3e} m")
+    assert err.max() < 1e-2
+
+
+@pytest.mark.parametrize("G", [1, 2, 4])
+def test_launch_shape_bit_identity(G):
+    announce(f"launch shape invariance G={G}")
+    w = inputs.workload("w2", n=700)
+    field = bqp_amd.kaula(10)
+    tf = np.minimum(w["tf"][:700], 4 * 3600.0)
+    with bqp_amd.Propagator(field, method="rk87", rtol=1e-11) as p:
+        p.set_launch(waves_per_group=1, sort_by_cost=0)
+        ref, s_ref, _ = p.propagate_batch_final(w["y0"][:700], 0.0, tf)
+        p.set_launch(waves_per_group=G, sort_by_cost=1, blocks=3)
+        got, s_got, _ = p.propagate_batch_final(w["y0"][:700], 0.0, tf)
+    assert np.array_equal(s_ref, s_got)
+    assert np.array_equal(ref, got)
+
+
+def test_batch_equals_single():
+    announce("batch row == single-row call")
+    w = inputs.workload("w0", n=130)
+    field = bqp_amd.kaula(8)
+    with bqp_amd.Propagator(field, method="dopri5", rtol=1e-10) as p:
+        yb, _, _ = p.propagate_batch_final(w["y0"], 0.0, 3600.0)
+        for i in (0, 64, 129):
+            ys, _, _ = p.propagate_batch_final(w["y0"][i:i + 1], 0.0, 3600.0)
+            assert np.array_equal(ys[0], yb[i])
+
+
+def test_failed_rows_nan_filled_with_status():
+    announce("row failure statuses")
+    field = bqp_amd.kaula(8)
+    w = inputs.workload("w0", n=8)
+    y0 = w["y0"].copy()
+    t0 = np.zeros(8)
+    tf = np.full(8, 1800.0)
+    y0[1, 2] = np.nan                 # invalid input
+    tf[2] = -5.0                      # tf < t0: invalid input
+    y0[3, :3] *= 0.5                  # starts below the minimum radius
+    tf[4] = 0.0                       # zero span: returns y0 exactly
+    with bqp_amd.Propagator(field, method="dopri5", rtol=1e-10, r_min=6378137.0) as p:
+        yf, st, info = p.propagate_batch_final(y0, t0, tf)
+    assert list(st) == [0, 5, 5, 6, 0, 0, 0, 0]
+    assert np.all(np.isnan(yf[1])) and np.all(np.isnan(yf[2])) and np.all(np.isnan(yf[3]))
+    assert np.array_equal(yf[4], y0[4])
+    assert info["rows_failed"] == 3 and info["first_failed_row"] == 1
+    with bqp_amd.Propagator(field, method="dopri5", rtol=1e-10, max_steps=5) as p:
+        yf, st, _ = p.propagate_batch_final(w["y0"][:3], 0.0, 86400.0)
+    assert np.all(st == 1) and np.all(np.isnan(yf))
+
+
+def test_empty_batch():
+    field = bqp_amd.kaula(4)
+    with bqp_amd.Propagator(field) as p:
+        yf, st, info = p.propagate_batch_final(np.zeros((0, 6)), np.zeros(0), np.zeros(0))
+    assert yf.shape == (0, 6) and st.shape == (0,)
+
+
+def test_sampled_matches_final():
+    announce("sampled grid vs final-state path")
+    w = inputs.workload("w0", n=32)
+    field = bqp_amd.kaula(8)
+    grid = np.array([0.0, 600.0, 1800.0, 5400.0])
+    with bqp_amd.Propagator(field, method="dop853", rtol=1e-12) as p:
+        out, st, _ = p.propagate_batch_sampled(w["y0"], 0.0, grid)
+        assert np.all(st == 0)
+        assert np.array_equal(out[:, 0], w["y0"])
+        for k, t in enumerate(grid[1:], start=1):
+            yf, _, _ = p.propagate_batch_final(w["y0"], 0.0, t)
+            assert pos_err(out[:, k], yf).max() < 1e-4
+        single, _, _ = p.propagate_batch_sampled(w["y0"], 0.0, grid[-1:])
+        yf, _, _ = p.propagate_batch_final(w["y0"], 0.0, grid[-1])
+        assert np.array_equal(single[:, 0], yf)
+
+
+def test_split_precision_trajectory_error():
+    """Split form vs FP64 GPU on 7-day N=70 arcs: final-position error <= 1 m."""
+    announce("split vs FP64 N=70 7 days 64 rows")
+    w = inputs.workload("w1", n=64)
+    field = bqp_amd.kaula(70)
+    with bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado") as p:
+        y64, s64, _ = p.propagate_batch_final(w["y0"], 0.0, w["tf"][:64], w["bc"][:64])
+        p.reconfigure(precision="split")
+        y32, s32, _ = p.propagate_batch_final(w["y0"], 0.0, w["tf"][:64], w["bc"][:64])
+    assert np.all(s64 == 0) and np.all(s32 == 0)
+    err = pos_err(y32, y64)
+    announce(f"split vs fp64 max |dr| = {err.max():.3e} m")
+    assert err.max() < 1.0

@@ -1,0 +1,146 @@
+"""CPU tests of the host side: seeded inputs, element conversions, the field
+loader, Python-boundary validation, and the C ABI's exported symbols (the
+library is loaded, no compute call is made without a GPU)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import bqp_amd
+from bqp_amd import _native, inputs
+from bqp_amd.elements import MU_EARTH, eci2kep, kep2eci
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+LIB = os.path.join(ROOT, "bqp_amd", "libbqp.so")
+
+
+def test_kaula_field_prefix_and_determinism():
+    C8, S8 = inputs.kaula_field(8)
+    C70, S70 = inputs.kaula_field(70)
+    k = inputs.tri_index(9, 0)
+    assert np.array_equal(C8, C70[:k]) and np.array_equal(S8, S70[:k])
+    C8b, _ = inputs.kaula_field(8)
+    assert np.array_equal(C8, C8b)
+    assert C8[0] == 1.0 and C8[1] == 0.0 and C8[2] == 0.0
+    assert C8[inputs.tri_index(2, 0)] == inputs.C20
+    # Kaula scale: std of degree-n coefficients ~ 1e-5 / n^2
+    n = 60
+    row = np.concatenate([C70[inputs.tri_index(n, 0):inputs.tri_index(n + 1, 0)],
+                          S70[inputs.tri_index(n, 1):inputs.tri_index(n + 1, 0)]])
+    assert 0.6 < np.std(row) / (1e-5 / n ** 2) < 1.4
+
+
+def test_workload_shapes_and_ranges():
+    w = inputs.workload("w1", n=2000)
+    r = np.linalg.norm(w["y0"][:, :3], axis=1)
+    assert w["y0"].shape == (2000, 6)
+    assert np.all(r >= (6378137.0 + 400e3) * 0.98) and np.all(r <= (6378137.0 + 1200e3) * 1.02)
+    assert np.all((w["bc"] >= 0.005) & (w["bc"] <= 0.02))
+    w2 = inputs.workload("w2", n=5000)
+    assert np.all(w2["tf"] >= 3600.0) and np.all(w2["tf"] <= 30 * 86400.0)
+    frac = np.bincount(w2["orbit_class"], minlength=3) / 5000
+    assert abs(frac[0] - 0.6) < 0.03 and abs(frac[1] - 0.25) < 0.03 and abs(frac[2] - 0.15) < 0.03
+    w4 = inputs.workload("w4", n=4096)
+    assert w4["generations"] * w4["population"] == 131072
+
+
+def test_kep2eci_matches_oracle(orc):
+    kep = inputs.leo_elements(50, seed=5)
+    a = kep2eci(kep)
+    b = orc.kep2eci(MU_EARTH, kep)
+    assert np.max(np.abs(a - b) / np.linalg.norm(a, axis=1)[:, None]) < 1e-13
+
+
+def test_eci2kep_round_trip_and_identities():
+    kep = inputs.leo_elements(200, seed=9)
+    s = kep2eci(kep)
+    back = eci2kep(s, anomaly="mean")
+    assert np.allclose(back[:, 0], kep[:, 0], rtol=1e-12)
+    assert np.allclose(back[:, 1], kep[:, 1], atol=1e-10)
+    assert np.allclose(back[:, 2], kep[:, 2], atol=1e-10)
+    r = np.linalg.norm(s[:, :3], axis=1)
+    v2 = np.sum(s[:, 3:] ** 2, axis=1)
+    assert np.allclose(v2, MU_EARTH * (2 / r - 1 / kep[:, 0]), rtol=1e-12)  # vis-viva
+    h = np.linalg.norm(np.cross(s[:, :3], s[:, 3:]), axis=1)
+    assert np.allclose(h, np.sqrt(MU_EARTH * kep[:, 0] * (1 - kep[:, 1] ** 2)), rtol=1e-12)
+
+
+def test_reference_shaped_kep_to_eci():
+    s = bqp_amd.kep_to_eci([7000.0, 0.001, 51.6, 10.0, 20.0, 30.0])
+    k = bqp_amd.eci_to_kep(s)
+    assert np.allclose(k, [7000.0, 0.001, 51.6, 10.0, 20.0, 30.0], rtol=1e-9, atol=1e-7)
+    with pytest.raises(ValueError):
+        bqp_amd.kep_to_eci([7000.0, 1.5, 0, 0, 0, 0])
+    with pytest.raises(ValueError):
+        bqp_amd.kep_to_eci([1, 2, 3])
+
+
+def test_goce_field_loader():
+    f = bqp_amd.goce(15)
+    assert f.nmax == 15 and f.cbar[0] == 1.0
+    assert f.cbar[inputs.tri_index(2, 0)] == pytest.approx(-4.84165301385e-4, rel=1e-12)
+    assert f.mu == 3.986004415e14 and f.radius == 6378136.46
+    t = f.truncated(5)
+    assert t.nmax == 5 and t.cbar.size == inputs.tri_index(6, 0)
+
+
+def test_icgem_gfc_parser(tmp_path):
+    p = tmp_path / "f.gfc"
+    p.write_text("product_type gravity_field\nearth_gravity_constant 3.986004415E+14\nradius 6378136.3\n"
+                 "max_degree 3\nnorm fully_normalized\nend_of_head ====\n"
+                 "gfc 2 0 -4.84D-04 0.0 1e-12 1e-12\ngfc 2 2 2.4e-06 -1.4e-06 0 0\ngfc 3 1 2.0e-06 2.5e-07 0 0\n")
+    f = bqp_amd.load_icgem(str(p))
+    assert f.nmax == 3 and f.radius == 6378136.3
+    assert f.cbar[inputs.tri_index(2, 0)] == -4.84e-4 and f.sbar[inputs.tri_index(2, 2)] == -1.4e-6
+
+
+def test_python_boundary_validation_before_device():
+    s = bqp_amd.kep_to_eci([7000.0, 0.001, 51.6, 10.0, 20.0, 30.0])
+    with pytest.raises(ValueError):
+        bqp_amd.propagate_final(s, 2451545.0, -1.0)
+    with pytest.raises(ValueError):
+        bqp_amd.propagate_final(s, 2451545.0, 600.0, srp=True)
+    with pytest.raises(ValueError):
+        bqp_amd.propagate_final(s, 2451545.0, 600.0, gravity_order=16)
+    with pytest.raises(ValueError):
+        bqp_amd.propagate_final(s, 2451545.0, 600.0, method="vern7")
+    with pytest.raises(ValueError):
+        bqp_amd.propagate_final(s, 2451545.0, 600.0, atm_model=7)
+    with pytest.raises(ValueError):
+        bqp_amd.propagate_final([1, 2, np.nan, 4, 5, 6], 2451545.0, 600.0)
+    with pytest.raises(ValueError):
+        bqp_amd.propagate(s, 2451545.0, 600.0, times_s=[100.0, 50.0])
+
+
+def _header_functions():
+    with open(os.path.join(ROOT, "include", "bqp.h")) as fh:
+        txt = fh.read()
+    return sorted(set(re.findall(r"\b(bqp_[a-z_0-9]+)\s*\(", txt)))
+
+
+def test_header_declares_exactly_the_bound_symbols():
+    assert _header_functions() == sorted(_native.EXPORTS)
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libbqp.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB)
+    for name in _header_functions():
+        assert hasattr(lib, name), name
+
+
+@pytest.mark.skipif(not os.path.exists(LIB), reason="libbqp.so not built")
+def test_host_only_entry_points_without_gpu():
+    L = _native.load()
+    cfg = _native.Config()
+    assert L.bqp_default_config(ctypes.byref(cfg)) == 0
+    assert cfg.method == 2 and cfg.rtol == 1e-12
+    assert L.bqp_default_config(None) == _native.E_INVALID
+    # null-context calls fail cleanly with E_INVALID, no device touched
+    assert L.bqp_destroy(None) == 0
+    assert L.bqp_load_field(None, 2, 2, None, None, 1.0, 1.0) == _native.E_INVALID
+    assert L.bqp_propagate_batch_final(None, None, None, None, None, 0, None, None, None, None, None) == \
+        _native.E_INVALID
+    assert L.bqp_last_error(None) == b"null context"
