@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--forms", default="fp64,split")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0)
+    ap.add_argument("--launch", default="", help='JSON launch shape, e.g. {"waves_per_group": 2}')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build's path runs on a single MI355X "
@@ -89,21 +90,19 @@ def main():
               f"see DESIGN.md).", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
     import bqp_amd
     from bqp_amd import inputs
+    from bqp_amd.device import DeviceArray, synchronize
 
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
     n = args.rows
     w = inputs.workload("w1", n=n)
     field = bqp_amd.kaula(70)
     tf_val = args.days * 86400.0
-    y0 = torch.from_numpy(w["y0"]).to(dev)
-    t0 = torch.zeros(n, dtype=torch.float64, device=dev)
-    tf = torch.full((n,), tf_val, dtype=torch.float64, device=dev)
-    bc = torch.from_numpy(w["bc"]).to(dev)
+    # inputs resident in HBM before any timed region
+    y0 = DeviceArray.from_numpy(w["y0"])
+    t0 = DeviceArray.from_numpy(np.zeros(n))
+    tf = DeviceArray.from_numpy(np.full(n, tf_val))
+    bc = DeviceArray.from_numpy(w["bc"])
     orbit_days = n * args.days
     fstage = stage_flops("rk87")
 
@@ -112,26 +111,28 @@ def main():
     finals = {}
     for form in forms:
         prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado", precision=form)
+        if args.launch:
+            prop.set_launch(**json.loads(args.launch))
         frhs = prop.flops_per_rhs
-        yf = torch.empty_like(y0)
-        status = torch.empty(n, dtype=torch.int32, device=dev)
-        steps_t = torch.empty(n, dtype=torch.int32, device=dev)
-        evals_t = torch.empty(n, dtype=torch.int32, device=dev)
+        yf = DeviceArray((n, 6))
+        status = DeviceArray((n,), np.int32)
+        steps_t = DeviceArray((n,), np.int32)
+        evals_t = DeviceArray((n,), np.int32)
         for _ in range(args.warmup):
             prop.propagate_batch_final_dev(y0, t0, tf, bc, yf, status, steps_t, evals_t)
-        torch.cuda.synchronize()
+        synchronize()
         wall = 0.0
         kern_ms = 0.0
         flops = 0.0
         for _ in range(args.steps):
-            torch.cuda.synchronize()
+            synchronize()
             ts = time.perf_counter()
             _, _, st = prop.propagate_batch_final_dev(y0, t0, tf, bc, yf, status, steps_t, evals_t)
-            torch.cuda.synchronize()
+            synchronize()
             wall += time.perf_counter() - ts
             kern_ms += st["kernel_ms"]
             flops += st["evals"] * frhs + (st["steps"] + st["rejects"]) * fstage
-        ok = int((status == 0).sum().item())
+        ok = int(np.sum(status.numpy() == 0))
         ms = wall / args.steps * 1e3
         kms = kern_ms / args.steps
         achieved = flops / args.steps / (kms * 1e-3) / 1e12
@@ -145,8 +146,9 @@ def main():
             "rejects": int(st["rejects"]),
             "flops_per_rhs": frhs,
             "achieved_tflops": achieved,
+            "launch": getattr(prop, "launch", None),
         }
-        finals[form] = yf.clone()
+        finals[form] = yf.numpy()
         prop.close()
         print(f"[bench] {form}: {results[form]['value']:.1f} orbit-days/s, {ms:.1f} ms/step, "
               f"kernel {kms:.1f} ms, {achieved:.2f} TFLOP/s, ok {ok}/{n}", file=sys.stderr, flush=True)
@@ -157,12 +159,13 @@ def main():
         for form in forms:
             if form == "fp64":
                 continue
-            d = torch.linalg.norm(finals[form][:, :3] - finals["fp64"][:, :3], dim=1)
+            d = np.linalg.norm(finals[form][:, :3] - finals["fp64"][:, :3], axis=1)
+            d = np.nan_to_num(d, nan=np.inf)
             forms_out[form] = {
                 "value": results[form]["value"],
                 "speedup_vs_fp64": results[form]["value"] / results["fp64"]["value"],
-                "max_pos_err_m_vs_fp64": float(torch.nan_to_num(d, nan=float("inf")).max().item()),
-                "median_pos_err_m_vs_fp64": float(torch.nan_to_num(d, nan=float("inf")).median().item()),
+                "max_pos_err_m_vs_fp64": float(d.max()),
+                "median_pos_err_m_vs_fp64": float(np.median(d)),
                 "achieved_tflops": results[form]["achieved_tflops"],
                 "kernel_ms": results[form]["kernel_ms"],
             }

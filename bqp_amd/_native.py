@@ -71,7 +71,7 @@ class Stats(C.Structure):
 class Launch(C.Structure):
     _fields_ = [
         ("waves_per_group", C.c_int32),
-        ("groups_per_block", C.c_int32),
+        ("use_graph", C.c_int32),
         ("blocks", C.c_int32),
         ("sort_by_cost", C.c_int32),
     ]

@@ -8,8 +8,6 @@
 // coefficient pack of .../src/config.rs:24-223, but is per context.
 #include <hip/hip_runtime.h>
 
-#include <rocprim/device/device_radix_sort.hpp>
-
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -22,6 +20,17 @@
 #include "bqp_kernels.hpp"
 
 using namespace bqp;
+
+// Identity of a captured launch sequence: a replay is valid only for the same
+// buffers, sizes, settings and field.
+struct GraphKey {
+  PropParams p;
+  int64_t n;
+  bqp_config cfg;
+  int G, blocks, sort;
+  const void* tab;
+  bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
+};
 
 struct bqp_ctx {
   bqp_config cfg{};
@@ -44,6 +53,9 @@ struct bqp_ctx {
   void* d_sort = nullptr;
   size_t sort_bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  GraphKey graph_key{};
+  int64_t graph_builds = 0;
   std::string err;
 };
 
@@ -211,46 +223,58 @@ int default_g(const bqp_ctx* c, int64_t n) {
   return 4;
 }
 
+// Enqueues the whole call (order, queue reset, persistent kernel, row
+// statistics) on the context stream.  Pure kernel launches: no allocation,
+// no memset, no synchronisation, so the sequence can be captured in a graph.
+void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, prop_fn kern, int G, int blocks,
+                         bool sort, bool events) {
+  PropParams q = p;
+  if (sort) {
+    char* base = (char*)c->d_sort;
+    unsigned int* hist = (unsigned int*)base;
+    int32_t* order = (int32_t*)(base + SORT_BUCKETS * sizeof(unsigned int));
+    unsigned short* bucket = (unsigned short*)(base + SORT_BUCKETS * sizeof(unsigned int) + n * sizeof(int32_t));
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(sort_clear_kernel, dim3(1), dim3(1024), 0, c->stream, hist);
+    hipLaunchKernelGGL(sort_hist_kernel, dim3(nb), dim3(256), 0, c->stream, p.y0, p.t0, p.tf, c->mu, (int)n,
+                       hist, bucket);
+    hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(1024), 0, c->stream, hist);
+    hipLaunchKernelGGL(sort_scatter_kernel, dim3(nb), dim3(256), 0, c->stream, bucket, (int)n, hist, order);
+    q.order = order;
+  }
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
+  if (events) (void)hipEventRecord(c->ev0, c->stream);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * G), 0, c->stream, q);
+  if (events) (void)hipEventRecord(c->ev1, c->stream);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_totals, 5);
+  hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, c->stream, c->d_first, 0x7fffffff);
+  hipLaunchKernelGGL(row_stats_kernel, dim3((unsigned)std::min<int64_t>(1024, (n + 255) / 256)), dim3(256), 0,
+                     c->stream, q.status, q.steps, q.evals, q.rejects, (int)n, c->d_totals);
+  hipLaunchKernelGGL(first_failed_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, q.status,
+                     (int)n, c->d_first);
+}
+
 // Runs one propagation over device buffers; fills per-row outputs.
 int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
   if (c->nmax < 0 || !c->d_tab) return fail(c, BQP_E_NO_FIELD, "no gravity field loaded (call bqp_load_field)");
   if (n <= 0) return BQP_OK;
   if (n > 0x7fffffffLL) return fail(c, BQP_E_INVALID, "n exceeds 2^31-1 rows");
   const bqp_config& cfg = c->cfg;
-  int G = c->launch.waves_per_group > 0 ? c->launch.waves_per_group : default_g(c, n);
+  const int G = c->launch.waves_per_group > 0 ? c->launch.waves_per_group : default_g(c, n);
   prop_fn kern = pick_kernel(cfg.method, cfg.precision, G);
   if (!kern) return fail(c, BQP_E_INVALID, "waves_per_group must be 1, 2 or 4");
   if (cfg.precision == BQP_PRECISION_SPLIT && !c->d_tabf)
     return fail(c, BQP_E_NO_FIELD, "split precision needs the FP32 table (reload the field)");
 
-  // optional longest-first order
-  int32_t* d_order = nullptr;
+  // all allocation happens here, before any launch is enqueued or captured
   const bool sort = c->launch.sort_by_cost != 0 && p.t_eval == nullptr && n > 64;
   if (sort) {
-    // layout in the sort scratch: keys_in, keys_out, idx_in, idx_out, temp
-    size_t temp = 0;
-    HIP_TRY(c, rocprim::radix_sort_pairs_desc(nullptr, temp, (float*)nullptr, (float*)nullptr,
-                                             (int32_t*)nullptr, (int32_t*)nullptr, (unsigned)n, 0, 32,
-                                             c->stream));
-    const size_t al = 256;
-    auto up = [&](size_t b) { return (b + al - 1) / al * al; };
-    const size_t need = 2 * up(n * sizeof(float)) + 2 * up(n * sizeof(int32_t)) + up(temp);
+    const size_t need = (size_t)SORT_BUCKETS * sizeof(unsigned int) + (size_t)n * sizeof(int32_t) +
+                        (size_t)n * sizeof(unsigned short) + 512;
     int rc = ensure(c, &c->d_sort, &c->sort_bytes, need);
     if (rc) return rc;
-    char* base = (char*)c->d_sort;
-    float* kin = (float*)base;
-    float* kout = (float*)(base + up(n * sizeof(float)));
-    int32_t* iin = (int32_t*)(base + 2 * up(n * sizeof(float)));
-    int32_t* iout = (int32_t*)(base + 2 * up(n * sizeof(float)) + up(n * sizeof(int32_t)));
-    void* tmp = base + 2 * up(n * sizeof(float)) + 2 * up(n * sizeof(int32_t));
-    hipLaunchKernelGGL(cost_key_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, p.y0,
-                       p.t0, p.tf, c->mu, (int)n, kin, iin);
-    HIP_TRY(c, hipGetLastError());
-    HIP_TRY(c, rocprim::radix_sort_pairs_desc(tmp, temp, kin, kout, iin, iout, (unsigned)n, 0, 32,
-                                             c->stream));
-    d_order = iout;
   }
-  p.order = d_order;
+  p.order = nullptr;
   p.queue = c->d_queue;
   p.n = (int)n;
   p.field = field_view(c);
@@ -279,20 +303,43 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
     blocks = (int)std::min<int64_t>(groups, (int64_t)per_cu * c->cus);
   }
   blocks = (int)std::max<int64_t>(1, std::min<int64_t>(blocks, groups));
-  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
-  HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * G), 0, c->stream, p);
-  HIP_TRY(c, hipGetLastError());
-  HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
-  if (stats || p.status) {
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_totals, 5);
-    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, c->stream, c->d_first, 0x7fffffff);
-    hipLaunchKernelGGL(row_stats_kernel, dim3(std::min<int64_t>(1024, (n + 255) / 256)), dim3(256), 0,
-                       c->stream, p.status, p.steps, p.evals, p.rejects, (int)n, c->d_totals);
-    hipLaunchKernelGGL(first_failed_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
-                       p.status, (int)n, c->d_first);
-    HIP_TRY(c, hipGetLastError());
+
+  if (c->launch.use_graph) {
+    GraphKey key;
+    std::memset(&key, 0, sizeof(key));
+    key.p = p;
+    key.n = n;
+    key.cfg = cfg;
+    key.G = G;
+    key.blocks = blocks;
+    key.sort = sort;
+    key.tab = c->d_tab;
+    if (!c->graph_exec || !(key == c->graph_key)) {
+      if (c->graph_exec) {
+        (void)hipGraphExecDestroy(c->graph_exec);
+        c->graph_exec = nullptr;
+      }
+      HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_propagation(c, p, n, kern, G, blocks, sort, false);
+      hipGraph_t graph = nullptr;
+      const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+      if (ce != hipSuccess || !graph) return fail(c, BQP_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
+      const hipError_t ie = hipGraphInstantiate(&c->graph_exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        c->graph_exec = nullptr;
+        return fail(c, BQP_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ie));
+      }
+      c->graph_key = key;
+      ++c->graph_builds;
+    }
+    HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(c, hipGraphLaunch(c->graph_exec, c->stream));
+    HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+  } else {
+    enqueue_propagation(c, p, n, kern, G, blocks, sort, true);
   }
+  HIP_TRY(c, hipGetLastError());
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   unsigned long long tot[5] = {0, 0, 0, 0, 0};
   int first = 0x7fffffff;
@@ -372,6 +419,15 @@ int32_t bqp_create(const bqp_config* cfg, bqp_ctx** out) {
   }
   c->cus = prop.multiProcessorCount;
   c->launch.sort_by_cost = 1;
+  {
+    static const bqp_tableau tabs[3] = {BQP_DOPRI5_TABLEAU, BQP_DOP853_TABLEAU, BQP_RK87_TABLEAU};
+    SecondOrderTableau so[3];
+    for (int m = 0; m < 3; ++m) make_second_order(tabs[m], so[m]);
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_so), so, sizeof(so)) != hipSuccess) {
+      bqp_destroy(c);
+      return BQP_E_DEVICE;
+    }
+  }
   *out = c;
   return BQP_OK;
 }
@@ -386,6 +442,7 @@ int32_t bqp_destroy(bqp_ctx* c) {
   (void)hipFree(c->d_first);
   (void)hipFree(c->d_stage);
   (void)hipFree(c->d_sort);
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -406,6 +463,7 @@ int32_t bqp_set_launch(bqp_ctx* c, const bqp_launch* l) {
   if (l->waves_per_group != 0 && l->waves_per_group != 1 && l->waves_per_group != 2 && l->waves_per_group != 4)
     return fail(c, BQP_E_INVALID, "waves_per_group must be 0 (auto), 1, 2 or 4");
   if (l->blocks < 0) return fail(c, BQP_E_INVALID, "blocks must be >= 0");
+  if (l->use_graph != 0 && l->use_graph != 1) return fail(c, BQP_E_INVALID, "use_graph must be 0 or 1");
   c->launch = *l;
   return BQP_OK;
 }

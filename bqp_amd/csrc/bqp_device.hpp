@@ -38,6 +38,18 @@ struct GravEntryF {
   float a, b, xv, xw, yv, yw, zv, zw;
 };
 
+// The table is read through the constant address space: loads from it are
+// invariant by definition, so uniform reads compile to scalar (SMEM) loads
+// even in kernels that also store to global memory.
+// (The host pass of hipcc parses device code without address spaces.)
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(4))) GravEntry* CEntry;
+typedef const __attribute__((address_space(4))) GravEntryF* CEntryF;
+#else
+typedef const GravEntry* CEntry;
+typedef const GravEntryF* CEntryF;
+#endif
+
 // Offset of column j in the entry table of a degree-N field.  Column 0 holds
 // n = 1..N+1 (N+1 entries); column j >= 1 holds n = j..N+1 (N+2-j entries).
 __host__ __device__ inline int col_offset(int N, int j) {
@@ -77,7 +89,7 @@ __device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, d
   double vs = R / sqrt(r2), ws = 0.0;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   const int top = f.nmax + 1;
-  const GravEntry* __restrict__ e = f.tab;
+  CEntry e = (CEntry)f.tab;
   if (col_owner<G>(0) == g) {
     double v1 = vs, v2 = 0.0;
     int n = 1;
@@ -193,8 +205,8 @@ __device__ __forceinline__ void grav_split(const FieldView& f, int g, double x, 
   double sx = 0.0, sy = 0.0, sz = 0.0;
   Comp32 cx = {0.f, 0.f}, cy = {0.f, 0.f}, cz = {0.f, 0.f};
   const int top = f.nmax + 1;
-  const GravEntry* __restrict__ e = f.tab;
-  const GravEntryF* __restrict__ ef = f.tabf;
+  CEntry e = (CEntry)f.tab;
+  CEntryF ef = (CEntryF)f.tabf;
   for (int j = 0; j <= f.jmax; ++j) {
     const int nlo = j == 0 ? 1 : j;
     const int len = top + 1 - nlo;

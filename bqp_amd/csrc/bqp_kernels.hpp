@@ -16,19 +16,67 @@
 //                     error sums)
 //   integrate_internal  solver.rs:1099-1925, PI controller macros :445-529
 //   compat preset     LY/src/odesolve/lightyear_compat.rs:100-600
-// Difference: after a rejected step k1 = f(t, y) is kept instead of being
-// re-evaluated (the value is bit-identical; only the eval count differs).
+// Differences: after a rejected step k1 = f(t, y) is kept instead of being
+// re-evaluated (the value is bit-identical; only the eval count differs); and
+// the stage sums use the second-order rearrangement of the same method.  For
+// y = (r, v), y' = (v, a(t, r, v)) the stage velocities are
+// V_s = v + h sum_j a_sj A_j, so
+//   R_s = r + h c^_s v + h^2 sum_j (A.A)_sj A_j        (c^_s = sum_j a_sj)
+//   r'  = r + h (sum b) v + h^2 sum_j (b.A)_j A_j,  v' = v + h sum_j b_j A_j
+// (and the same for the error weights).  Only the stage accelerations A_j
+// (3 doubles per stage, not 6) stay live, which halves the register
+// footprint of the 13-stage method.
 #pragma once
 
 #include "bqp_device.hpp"
 
 #ifndef BQP_WAVES_PER_EU
-#define BQP_WAVES_PER_EU 1
+#define BQP_WAVES_PER_EU 2
 #endif
 
 namespace bqp {
 
 __constant__ bqp_tableau c_tab[3] = {BQP_DOPRI5_TABLEAU, BQP_DOP853_TABLEAU, BQP_RK87_TABLEAU};
+
+// Second-order rearrangement of each tableau, computed on the host
+// (make_second_order) and copied to the device when a context is created.
+struct SecondOrderTableau {
+  double csum[BQP_MAX_STAGES];                  // sum_j a_sj
+  double a2[BQP_MAX_STAGES][BQP_MAX_STAGES];    // (A.A)_sj
+  double ba[BQP_MAX_STAGES];                    // (b.A)_j
+  double ea[BQP_MAX_STAGES];                    // (e.A)_j
+  double e3a[BQP_MAX_STAGES];                   // (e3.A)_j
+  double bsum, esum, e3sum;
+};
+__constant__ SecondOrderTableau c_so[3];
+
+inline void make_second_order(const bqp_tableau& t, SecondOrderTableau& o) {
+  const int S = t.stages;
+  o = SecondOrderTableau{};
+  long double bs = 0, es = 0, e3s = 0;
+  for (int i = 0; i < S; ++i) {
+    long double cs = 0, ba = 0, ea = 0, e3a = 0;
+    for (int j = 0; j < S; ++j) {
+      cs += (long double)t.a[i][j];
+      long double a2 = 0;
+      for (int k = 0; k < S; ++k) a2 += (long double)t.a[i][k] * (long double)t.a[k][j];
+      o.a2[i][j] = (double)a2;
+      ba += (long double)t.b[j] * (long double)t.a[j][i];
+      ea += (long double)t.e[j] * (long double)t.a[j][i];
+      e3a += (long double)t.e3[j] * (long double)t.a[j][i];
+    }
+    o.csum[i] = (double)cs;
+    o.ba[i] = (double)ba;
+    o.ea[i] = (double)ea;
+    o.e3a[i] = (double)e3a;
+    bs += t.b[i];
+    es += t.e[i];
+    e3s += t.e3[i];
+  }
+  o.bsum = (double)bs;
+  o.esum = (double)es;
+  o.e3sum = (double)e3s;
+}
 
 template <int METHOD>
 struct MethodTraits;
@@ -101,9 +149,9 @@ __device__ __forceinline__ int lane_rank(unsigned long long mask) {
 // Inertial Cowell RHS: uniform z-rotation to the body frame, gravity, rotate
 // back, exponential drag (LY/src/rhs.rs:1119-1163 compute_drag).
 template <int PREC, int G>
-__device__ __forceinline__ void rhs_eval(const PropParams& p, int g, int lane,
-                                         double (*lds)[3][64], double bc, double t,
-                                         const double* y, double* dy) {
+__device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
+                                           double (*lds)[3][64], double bc, double t,
+                                           const double* y, double* acc) {
   const double th = fma(p.model.omega, t, p.model.theta0);
   double s, c;
   sincos(th, &s, &c);
@@ -148,12 +196,9 @@ __device__ __forceinline__ void rhs_eval(const PropParams& p, int g, int lane,
       az = fma(k, vrz, az);
     }
   }
-  dy[0] = y[3];
-  dy[1] = y[4];
-  dy[2] = y[5];
-  dy[3] = ax;
-  dy[4] = ay;
-  dy[5] = az;
+  acc[0] = ax;
+  acc[1] = ay;
+  acc[2] = az;
 }
 
 template <int METHOD, int PREC, int G>
@@ -163,7 +208,10 @@ propagate_kernel(const PropParams p) {
   constexpr int S = MT::S;
   const bqp_tableau& T = c_tab[METHOD];
   const int lane = threadIdx.x & 63;
-  const int g = threadIdx.x >> 6;
+  // wave index within the group; readfirstlane tells the compiler it is
+  // wave-uniform, so column ownership stays a scalar branch and the
+  // coefficient entries stay scalar loads
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   __shared__ double lds[G][3][64];
   __shared__ unsigned int lds_base;
 
@@ -178,11 +226,12 @@ propagate_kernel(const PropParams p) {
   double hmin = 0.0, hmax = 0.0, eps = p.eps;
   bool have_prev = false, just_rej = false, have_k1 = false;
   double y[6] = {0, 0, 0, 0, 0, 0};
-  double k[S][6];
+  const SecondOrderTableau& SO = c_so[METHOD];
+  double A[S][3];  // stage accelerations
 #pragma unroll
   for (int j = 0; j < S; ++j)
 #pragma unroll
-    for (int i = 0; i < 6; ++i) k[j][i] = 0.0;
+    for (int i = 0; i < 3; ++i) A[j][i] = 0.0;
 
   // Writes the lane's row result and frees the lane.
   auto finish = [&](int st) {
@@ -298,83 +347,112 @@ propagate_kernel(const PropParams p) {
     {
       const bool needk = stepping && !have_k1;
       if (wave_ballot(needk) != 0ull) {
-        double kk[6];
-        rhs_eval<PREC, G>(p, g, lane, lds, bc, t, y, kk);
+        double a0[3];
+        accel_eval<PREC, G>(p, g, lane, lds, bc, t, y, a0);
         if (needk) {
 #pragma unroll
-          for (int i = 0; i < 6; ++i) k[0][i] = kk[i];
+          for (int i = 0; i < 3; ++i) A[0][i] = a0[i];
           ++evals;
           have_k1 = true;
         }
       }
     }
     if (wave_ballot(stepping) == 0ull) continue;
+    const double hh = hs * hs;
 #pragma clang loop unroll(disable)
     for (int s = 1; s < S; ++s) {
       double ys[6];
+      const double hc = hs * SO.csum[s];
 #pragma unroll
-      for (int i = 0; i < 6; ++i) ys[i] = y[i];
+      for (int i = 0; i < 3; ++i) {
+        ys[i] = fma(y[3 + i], hc, y[i]);
+        ys[3 + i] = y[3 + i];
+      }
 #pragma unroll
       for (int j = 0; j < S - 1; ++j) {
         if (j < s) {
           const double a = T.a[s][j];
           if (a != 0.0) {
-            const double sc = hs * a;
+            const double sv = hs * a;
 #pragma unroll
-            for (int i = 0; i < 6; ++i) ys[i] = fma(k[j][i], sc, ys[i]);
+            for (int i = 0; i < 3; ++i) ys[3 + i] = fma(A[j][i], sv, ys[3 + i]);
+          }
+          const double a2 = SO.a2[s][j];
+          if (a2 != 0.0) {
+            const double sr = hh * a2;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) ys[i] = fma(A[j][i], sr, ys[i]);
           }
         }
       }
-      double kk[6];
-      rhs_eval<PREC, G>(p, g, lane, lds, bc, t + T.c[s] * hs, ys, kk);
+      double as[3];
+      accel_eval<PREC, G>(p, g, lane, lds, bc, t + T.c[s] * hs, ys, as);
 #pragma unroll
       for (int j = 1; j < S; ++j)
         if (j == s) {
 #pragma unroll
-          for (int i = 0; i < 6; ++i) k[j][i] = kk[i];
+          for (int i = 0; i < 3; ++i) A[j][i] = as[i];
         }
       if (stepping) ++evals;
     }
     if (!stepping) continue;
 
-    // solution and error estimates
+    // solution and error estimates (Kahan-compensated error sums)
     double yn[6], err[6], comp[6], err3[6], comp3[6];
+    {
+      const double hb = hs * SO.bsum, he = hs * SO.esum, he3 = hs * SO.e3sum;
 #pragma unroll
-    for (int i = 0; i < 6; ++i) {
-      yn[i] = y[i];
-      err[i] = comp[i] = err3[i] = comp3[i] = 0.0;
+      for (int i = 0; i < 3; ++i) {
+        yn[i] = fma(y[3 + i], hb, y[i]);
+        yn[3 + i] = y[3 + i];
+        err[i] = he * y[3 + i];
+        err[3 + i] = 0.0;
+        err3[i] = MT::E3 ? he3 * y[3 + i] : 0.0;
+        err3[3 + i] = 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < 6; ++i) comp[i] = comp3[i] = 0.0;
     }
+    auto kahan = [](double& sum, double& c, double term) {
+      const double yk = term - c;
+      const double acc = sum + yk;
+      c = (acc - sum) - yk;
+      sum = acc;
+    };
 #pragma unroll
     for (int j = 0; j < S; ++j) {
-      const double bj = T.b[j], ej = T.e[j];
+      const double bj = T.b[j], ej = T.e[j], baj = SO.ba[j], eaj = SO.ea[j];
       if (bj != 0.0) {
-        const double hb = hs * bj;
+        const double hbj = hs * bj;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) yn[i] = fma(k[j][i], hb, yn[i]);
+        for (int i = 0; i < 3; ++i) yn[3 + i] = fma(A[j][i], hbj, yn[3 + i]);
+      }
+      if (baj != 0.0) {
+        const double hba = hh * baj;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) yn[i] = fma(A[j][i], hba, yn[i]);
       }
       if (ej != 0.0) {
-        const double he = hs * ej;
+        const double hej = hs * ej;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) {
-          const double term = he * k[j][i];
-          const double yk = term - comp[i];
-          const double acc = err[i] + yk;
-          comp[i] = (acc - err[i]) - yk;
-          err[i] = acc;
-        }
+        for (int i = 0; i < 3; ++i) kahan(err[3 + i], comp[3 + i], hej * A[j][i]);
+      }
+      if (eaj != 0.0) {
+        const double hea = hh * eaj;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) kahan(err[i], comp[i], hea * A[j][i]);
       }
       if (MT::E3) {
-        const double e3 = T.e3[j];
-        if (e3 != 0.0) {
-          const double he3 = hs * e3;
+        const double e3j = T.e3[j], e3aj = SO.e3a[j];
+        if (e3j != 0.0) {
+          const double h3 = hs * e3j;
 #pragma unroll
-          for (int i = 0; i < 6; ++i) {
-            const double term = he3 * k[j][i];
-            const double yk = term - comp3[i];
-            const double acc = err3[i] + yk;
-            comp3[i] = (acc - err3[i]) - yk;
-            err3[i] = acc;
-          }
+          for (int i = 0; i < 3; ++i) kahan(err3[3 + i], comp3[3 + i], h3 * A[j][i]);
+        }
+        if (e3aj != 0.0) {
+          const double h3a = hh * e3aj;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) kahan(err3[i], comp3[i], h3a * A[j][i]);
         }
       }
     }
@@ -440,7 +518,7 @@ propagate_kernel(const PropParams p) {
       rej_run = 0;
       if (MT::FSAL) {
 #pragma unroll
-        for (int i = 0; i < 6; ++i) k[0][i] = k[S - 1][i];
+        for (int i = 0; i < 3; ++i) A[0][i] = A[S - 1][i];
         have_k1 = true;
       } else {
         have_k1 = false;
@@ -532,25 +610,76 @@ __global__ void __launch_bounds__(256) gravity_kernel(FieldView f, const double*
 // Queue reset without hipMemsetAsync (graph-capture safe).
 __global__ void reset_counter_kernel(unsigned int* q) { q[0] = 0u; }
 
-// Cost key per row for longest-first hand-out: orbit revolutions over the
-// span (span * mean motion from vis-viva), so rows of similar length share
-// a wave.
-__global__ void cost_key_kernel(const double* __restrict__ y0, const double* __restrict__ t0,
-                                const double* __restrict__ tf, double mu, int n,
-                                float* __restrict__ key, int32_t* __restrict__ idx) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double* s = y0 + 6 * (size_t)i;
+// Longest-first hand-out order (a counting sort made of plain kernels, so it
+// can be captured in a HIP graph).  The cost key is the number of orbit
+// revolutions over the span (span * mean motion from vis-viva); at a fixed
+// tolerance the step count per revolution varies slowly, so rows with similar
+// keys take similar step counts and share a wave.  Row results do not depend
+// on which lane runs them, so any tie order is exact.
+constexpr int SORT_BUCKETS = 4096;
+
+__device__ __forceinline__ int cost_bucket(const double* __restrict__ s, double span, double mu) {
   const double r = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
   const double v2 = s[3] * s[3] + s[4] * s[4] + s[5] * s[5];
   const double en = 0.5 * v2 - mu / r;
   double rev = 0.0;
-  if (en < 0.0 && r > 0.0) {
+  if (en < 0.0 && r > 0.0 && span > 0.0) {
     const double a = -mu / (2.0 * en);
-    rev = (tf[i] - t0[i]) * sqrt(mu / (a * a * a));
+    rev = span * sqrt(mu / (a * a * a));
   }
-  key[i] = isfinite(rev) ? (float)rev : 0.0f;
-  idx[i] = i;
+  // 128 buckets per octave of revolutions, most expensive first
+  int b = isfinite(rev) && rev > 0.0 ? (int)(log2(1.0 + rev) * 128.0) : 0;
+  b = b < 0 ? 0 : (b > SORT_BUCKETS - 1 ? SORT_BUCKETS - 1 : b);
+  return SORT_BUCKETS - 1 - b;
+}
+
+__global__ void sort_clear_kernel(unsigned int* __restrict__ hist) {
+  for (int i = threadIdx.x; i < SORT_BUCKETS; i += blockDim.x) hist[i] = 0u;
+}
+
+__global__ void sort_hist_kernel(const double* __restrict__ y0, const double* __restrict__ t0,
+                                 const double* __restrict__ tf, double mu, int n,
+                                 unsigned int* __restrict__ hist, unsigned short* __restrict__ bucket) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = cost_bucket(y0 + 6 * (size_t)i, tf[i] - t0[i], mu);
+  bucket[i] = (unsigned short)b;
+  atomicAdd(&hist[b], 1u);
+}
+
+// exclusive scan of the histogram in place (one block of 1024 threads)
+__global__ void __launch_bounds__(1024) sort_scan_kernel(unsigned int* __restrict__ hist) {
+  __shared__ unsigned int part[1024];
+  constexpr int PER = SORT_BUCKETS / 1024;
+  const int t = threadIdx.x;
+  unsigned int loc[PER], sum = 0;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    loc[k] = hist[t * PER + k];
+    sum += loc[k];
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const unsigned int v = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  unsigned int run = part[t] - sum;
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    hist[t * PER + k] = run;
+    run += loc[k];
+  }
+}
+
+__global__ void sort_scatter_kernel(const unsigned short* __restrict__ bucket, int n,
+                                    unsigned int* __restrict__ offs, int32_t* __restrict__ order) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned int pos = atomicAdd(&offs[bucket[i]], 1u);
+  order[pos] = i;
 }
 
 // Sum of per-row counters into 64-bit totals (stats).

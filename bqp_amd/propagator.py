@@ -8,7 +8,7 @@ field) and exposes the batch call shapes:
   (dust_transfer/crates/lightyear_odeint_rs/src/session.rs:2325-2436);
 * ``propagate_batch_sampled`` fixed sample grid -> (n, m, 6), the shape of
   ``integrate_batch_native_into`` (.../src/batch.rs:247-390) in inertial states;
-* ``propagate_batch_final_dev`` the same on device-resident torch tensors;
+* ``propagate_batch_final_dev`` the same on device-resident buffers;
 * ``gravity_accel``          body-fixed accelerations
   (crates/satpy_core/src/gravity.rs:2638 ``spherical_gravity_impl_frame_packed``).
 
@@ -117,9 +117,12 @@ class Propagator:
     def __exit__(self, *exc):
         self.close()
 
-    def set_launch(self, waves_per_group: int = 0, groups_per_block: int = 0, blocks: int = 0,
-                   sort_by_cost: int = 1):
-        L = nat.Launch(waves_per_group, groups_per_block, blocks, sort_by_cost)
+    def set_launch(self, waves_per_group: int = 0, use_graph: int = 0, blocks: int = 0, sort_by_cost: int = 1):
+        """Launch shape: waves per trajectory group G (0 = auto), HIP-graph
+        replay, persistent grid size (0 = fill the device), cost-sorted hand-out."""
+        L = nat.Launch(waves_per_group, use_graph, blocks, sort_by_cost)
+        self.launch = {"waves_per_group": waves_per_group, "use_graph": use_graph, "blocks": blocks,
+                       "sort_by_cost": sort_by_cost}
         self._check(self._lib.bqp_set_launch(self._ctx, C.byref(L)))
 
     def reconfigure(self, **kw):
@@ -174,25 +177,23 @@ class Propagator:
         return yf, status, info
 
     def propagate_batch_final_dev(self, y0, t0, tf, bc=None, yf=None, status=None, steps=None, evals=None):
-        """Device-resident variant on torch CUDA tensors (float64 / int32);
-        returns (yf, status, stats dict).  No host staging."""
-        import torch
+        """Device-resident variant: arguments are device buffers exposing
+        ``data_ptr()`` (``bqp_amd.device.DeviceArray``; float64 states and
+        times, int32 counters).  No host staging.  Returns (yf, status, stats)."""
+        from .device import DeviceArray
 
-        n = y0.shape[0]
-        for name, t in (("y0", y0), ("t0", t0), ("tf", tf)):
-            if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
-                raise ValueError(f"{name} must be a contiguous float64 device tensor")
+        n = int(y0.shape[0])
         if yf is None:
-            yf = torch.empty_like(y0)
+            yf = DeviceArray((n, 6), np.float64)
         if status is None:
-            status = torch.empty(n, dtype=torch.int32, device=y0.device)
+            status = DeviceArray((n,), np.int32)
         st = nat.Stats()
-        torch.cuda.synchronize(y0.device)
-        rc = self._lib.bqp_propagate_batch_final_dev(
-            self._ctx, C.c_void_p(y0.data_ptr()), C.c_void_p(t0.data_ptr()), C.c_void_p(tf.data_ptr()),
-            C.c_void_p(bc.data_ptr()) if bc is not None else None, n, C.c_void_p(yf.data_ptr()),
-            C.c_void_p(status.data_ptr()), C.c_void_p(steps.data_ptr()) if steps is not None else None,
-            C.c_void_p(evals.data_ptr()) if evals is not None else None, C.byref(st))
+
+        def ptr(x):
+            return C.c_void_p(x.data_ptr()) if x is not None else None
+
+        rc = self._lib.bqp_propagate_batch_final_dev(self._ctx, ptr(y0), ptr(t0), ptr(tf), ptr(bc), n, ptr(yf),
+                                                     ptr(status), ptr(steps), ptr(evals), C.byref(st))
         self._check(rc, allow_rows=True)
         return yf, status, st.as_dict()
 

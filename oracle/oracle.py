@@ -139,7 +139,9 @@ def density(model: str, h_m: float) -> float:
 
 def make_model(field: Field, omega=7.292115e-5, theta0=0.0, drag="none", r_atm=6378137.0,
                r_min=0.0) -> Model:
-    return Model(field.ptr, omega, theta0, DRAG[drag], r_atm, r_min)
+    m = Model(field.ptr, omega, theta0, DRAG[drag], r_atm, r_min)
+    m._field_ref = field  # the C field must outlive every model that points at it
+    return m
 
 
 def make_integ(method="rk87", control="scaled", rtol=1e-12, atol_pos=None, atol_vel=None,
