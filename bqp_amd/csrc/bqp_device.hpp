@@ -77,8 +77,33 @@ __device__ __forceinline__ int col_owner(int j) {
   return (k & 1) ? (G - 1 - r) : r;
 }
 
+// Sectoral factors s_1 = sqrt(3), s_j = sqrt((2j+1)/(2j)) depend only on j;
+// they live in constant memory (filled at context creation) so the sectoral
+// chain every wave walks reads a few hundred bytes that stay in the scalar
+// cache, not the first entry of every column.
+constexpr int SECT_MAX = 208;
+__constant__ double c_sect[SECT_MAX];
+__constant__ float c_sectf[SECT_MAX];
+
+// One recursion step of a column: V(n) from V(n-1), V(n-2), then the
+// contributions of entry e.  Writes the new value over V(n-2)'s register so a
+// pair of steps needs no moves.
+#define BQP_COL_STEP(E, VOUT, VPREV, WOUT, WPREV)   \
+  {                                                 \
+    const double A_ = (E).a * zt, B_ = (E).b * rho; \
+    VOUT = fma(A_, VPREV, -(B_ * VOUT));            \
+    WOUT = fma(A_, WPREV, -(B_ * WOUT));            \
+    sx = fma((E).xv, VOUT, sx);                     \
+    sx = fma((E).xw, WOUT, sx);                     \
+    sy = fma((E).yv, VOUT, sy);                     \
+    sy = fma((E).yw, WOUT, sy);                     \
+    sz = fma((E).zv, VOUT, sz);                     \
+    sz = fma((E).zw, WOUT, sz);                     \
+  }
+
 // FP64 gravity, body-fixed position (m) -> acceleration (m/s^2), for the
 // columns owned by wave g of G.  The caller sums the G partial results.
+// Recursion entries are consumed four per scalar-load batch.
 template <int G>
 __device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, double y, double z,
                                           double& ax, double& ay, double& az) {
@@ -91,86 +116,67 @@ __device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, d
   const int top = f.nmax + 1;
   CEntry e = (CEntry)f.tab;
   if (col_owner<G>(0) == g) {
-    double v1 = vs, v2 = 0.0;
-    int n = 1;
-    // two entries per trip: the pair's results land in (v2, v1) without moves
-    for (; n + 1 <= top; n += 2, e += 2) {
-      const GravEntry e0 = e[0], e1 = e[1];
-      const double A0 = e0.a * zt, B0 = e0.b * rho;
-      v2 = fma(A0, v1, -(B0 * v2));
-      sx = fma(e0.xv, v2, sx);
-      sy = fma(e0.yv, v2, sy);
-      sz = fma(e0.zv, v2, sz);
-      const double A1 = e1.a * zt, B1 = e1.b * rho;
-      v1 = fma(A1, v2, -(B1 * v1));
-      sx = fma(e1.xv, v1, sx);
-      sy = fma(e1.yv, v1, sy);
-      sz = fma(e1.zv, v1, sz);
+    // column 0: W = 0 throughout
+    double v1 = vs, v2 = 0.0, w1 = 0.0, w2 = 0.0;
+    CEntry r = e;
+    int k = top;
+    for (; k >= 4; k -= 4, r += 4) {
+      const GravEntry e0 = r[0], e1 = r[1], e2 = r[2], e3 = r[3];
+      BQP_COL_STEP(e0, v2, v1, w2, w1);
+      BQP_COL_STEP(e1, v1, v2, w1, w2);
+      BQP_COL_STEP(e2, v2, v1, w2, w1);
+      BQP_COL_STEP(e3, v1, v2, w1, w2);
     }
-    if (n <= top) {
-      const double A = e->a * zt, B = e->b * rho;
-      const double v = fma(A, v1, -(B * v2));
-      sx = fma(e->xv, v, sx);
-      sy = fma(e->yv, v, sy);
-      sz = fma(e->zv, v, sz);
-      ++e;
+    for (; k >= 2; k -= 2, r += 2) {
+      const GravEntry e0 = r[0], e1 = r[1];
+      BQP_COL_STEP(e0, v2, v1, w2, w1);
+      BQP_COL_STEP(e1, v1, v2, w1, w2);
     }
-  } else {
-    e += top;
+    if (k) {
+      const GravEntry e0 = r[0];
+      BQP_COL_STEP(e0, v2, v1, w2, w1);
+    }
   }
+  e += top;
   for (int j = 1; j <= f.jmax; ++j) {
-    const double s = e->a;
+    const double s = c_sect[j];
     const double nv = s * fma(xt, vs, -(yt * ws));
     const double nw = s * fma(xt, ws, yt * vs);
     vs = nv;
     ws = nw;
     const int len = top + 1 - j;
     if (col_owner<G>(j) == g) {
-      sx = fma(e->xv, vs, sx);
-      sx = fma(e->xw, ws, sx);
-      sy = fma(e->yv, vs, sy);
-      sy = fma(e->yw, ws, sy);
-      sz = fma(e->zv, vs, sz);
-      sz = fma(e->zw, ws, sz);
-      ++e;
       double v1 = vs, w1 = ws, v2 = 0.0, w2 = 0.0;
-      int k = 1;
-      for (; k + 1 < len; k += 2, e += 2) {
-        const GravEntry e0 = e[0], e1 = e[1];
-        const double A0 = e0.a * zt, B0 = e0.b * rho;
-        v2 = fma(A0, v1, -(B0 * v2));
-        w2 = fma(A0, w1, -(B0 * w2));
-        sx = fma(e0.xv, v2, sx);
-        sx = fma(e0.xw, w2, sx);
-        sy = fma(e0.yv, v2, sy);
-        sy = fma(e0.yw, w2, sy);
-        sz = fma(e0.zv, v2, sz);
-        sz = fma(e0.zw, w2, sz);
-        const double A1 = e1.a * zt, B1 = e1.b * rho;
-        v1 = fma(A1, v2, -(B1 * v1));
-        w1 = fma(A1, w2, -(B1 * w1));
-        sx = fma(e1.xv, v1, sx);
-        sx = fma(e1.xw, w1, sx);
-        sy = fma(e1.yv, v1, sy);
-        sy = fma(e1.yw, w1, sy);
-        sz = fma(e1.zv, v1, sz);
-        sz = fma(e1.zw, w1, sz);
+      CEntry r = e + 1;
+      int k = len - 1;
+      {
+        // sectoral entry, loaded with the first batch of recursion entries
+        const GravEntry es = e[0];
+        sx = fma(es.xv, vs, sx);
+        sx = fma(es.xw, ws, sx);
+        sy = fma(es.yv, vs, sy);
+        sy = fma(es.yw, ws, sy);
+        sz = fma(es.zv, vs, sz);
+        sz = fma(es.zw, ws, sz);
       }
-      if (k < len) {
-        const double A = e->a * zt, B = e->b * rho;
-        const double v = fma(A, v1, -(B * v2));
-        const double w = fma(A, w1, -(B * w2));
-        sx = fma(e->xv, v, sx);
-        sx = fma(e->xw, w, sx);
-        sy = fma(e->yv, v, sy);
-        sy = fma(e->yw, w, sy);
-        sz = fma(e->zv, v, sz);
-        sz = fma(e->zw, w, sz);
-        ++e;
+      for (; k >= 4; k -= 4, r += 4) {
+        const GravEntry e0 = r[0], e1 = r[1], e2 = r[2], e3 = r[3];
+        BQP_COL_STEP(e0, v2, v1, w2, w1);
+        BQP_COL_STEP(e1, v1, v2, w1, w2);
+        BQP_COL_STEP(e2, v2, v1, w2, w1);
+        BQP_COL_STEP(e3, v1, v2, w1, w2);
       }
-    } else {
-      e += len;
+      for (; k >= 2; k -= 2, r += 2) {
+        const GravEntry e0 = r[0], e1 = r[1];
+        BQP_COL_STEP(e0, v2, v1, w2, w1);
+        BQP_COL_STEP(e1, v1, v2, w1, w2);
+      }
+      if (k) {
+        const GravEntry e0 = r[0];
+        BQP_COL_STEP(e0, v2, v1, w2, w1);
+      }
     }
+    e += len;
   }
   ax = sx;
   ay = sy;
@@ -212,7 +218,7 @@ __device__ __forceinline__ void grav_split(const FieldView& f, int g, double x, 
     const int len = top + 1 - nlo;
     if (j > 0) {
       if (j <= SPLIT_N) {
-        const double s = e->a;
+        const double s = c_sect[j];
         const double nv = s * fma(xt, vs, -(yt * ws));
         const double nw = s * fma(xt, ws, yt * vs);
         vs = nv;
@@ -222,7 +228,7 @@ __device__ __forceinline__ void grav_split(const FieldView& f, int g, double x, 
           wsf = (float)ws;
         }
       } else {
-        const float s = ef->a;
+        const float s = c_sectf[j];
         const float nv = s * fmaf(xtf, vsf, -(ytf * wsf));
         const float nw = s * fmaf(xtf, wsf, ytf * vsf);
         vsf = nv;

@@ -31,7 +31,7 @@
 #include "bqp_device.hpp"
 
 #ifndef BQP_WAVES_PER_EU
-#define BQP_WAVES_PER_EU 2
+#define BQP_WAVES_PER_EU 4
 #endif
 
 namespace bqp {
@@ -147,11 +147,16 @@ __device__ __forceinline__ int lane_rank(unsigned long long mask) {
 }
 
 // Inertial Cowell RHS: uniform z-rotation to the body frame, gravity, rotate
-// back, exponential drag (LY/src/rhs.rs:1119-1163 compute_drag).
-template <int PREC, int G>
+// back, exponential drag (LY/src/rhs.rs:1119-1163 compute_drag).  The
+// acceleration is computed by wave 0 only and, for lanes with `store`,
+// written into the group's LDS stage slot sA[slot]; with G > 1 the trailing
+// barrier both frees the partial-sum buffer and publishes sA[slot] to the
+// other waves, which only contribute gravity partial sums.
+template <int PREC, int G, int S>
 __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
-                                           double (*lds)[3][64], double bc, double t,
-                                           const double* y, double* acc) {
+                                           double (*lds)[3][64], double (*sA)[3][64], int slot,
+                                           bool store, double bc, double t, const double* y,
+                                           double* acc) {
   const double th = fma(p.model.omega, t, p.model.theta0);
   double s, c;
   sincos(th, &s, &c);
@@ -177,7 +182,12 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
       gy += lds[q][1][lane];
       gz += lds[q][2][lane];
     }
+  }
+  // only wave 0 publishes the stage acceleration; the other waves' copies
+  // would be identical, so they skip the inertial rotation and the drag
+  if (G > 1 && g != 0) {
     __syncthreads();
+    return;
   }
   double ax = fma(c, gx, -(s * gy));
   double ay = fma(s, gx, c * gy);
@@ -199,6 +209,12 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
   acc[0] = ax;
   acc[1] = ay;
   acc[2] = az;
+  if (store) {
+    sA[slot][0][lane] = ax;
+    sA[slot][1][lane] = ay;
+    sA[slot][2][lane] = az;
+  }
+  if (G > 1) __syncthreads();
 }
 
 template <int METHOD, int PREC, int G>
@@ -227,11 +243,8 @@ propagate_kernel(const PropParams p) {
   bool have_prev = false, just_rej = false, have_k1 = false;
   double y[6] = {0, 0, 0, 0, 0, 0};
   const SecondOrderTableau& SO = c_so[METHOD];
-  double A[S][3];  // stage accelerations
-#pragma unroll
-  for (int j = 0; j < S; ++j)
-#pragma unroll
-    for (int i = 0; i < 3; ++i) A[j][i] = 0.0;
+  // stage accelerations of the group's 64 trajectories, shared by its G waves
+  __shared__ double sA[S][3][64];
 
   // Writes the lane's row result and frees the lane.
   auto finish = [&](int st) {
@@ -348,10 +361,8 @@ propagate_kernel(const PropParams p) {
       const bool needk = stepping && !have_k1;
       if (wave_ballot(needk) != 0ull) {
         double a0[3];
-        accel_eval<PREC, G>(p, g, lane, lds, bc, t, y, a0);
+        accel_eval<PREC, G, S>(p, g, lane, lds, sA, 0, needk, bc, t, y, a0);
         if (needk) {
-#pragma unroll
-          for (int i = 0; i < 3; ++i) A[0][i] = a0[i];
           ++evals;
           have_k1 = true;
         }
@@ -375,24 +386,18 @@ propagate_kernel(const PropParams p) {
           if (a != 0.0) {
             const double sv = hs * a;
 #pragma unroll
-            for (int i = 0; i < 3; ++i) ys[3 + i] = fma(A[j][i], sv, ys[3 + i]);
+            for (int i = 0; i < 3; ++i) ys[3 + i] = fma(sA[j][i][lane], sv, ys[3 + i]);
           }
           const double a2 = SO.a2[s][j];
           if (a2 != 0.0) {
             const double sr = hh * a2;
 #pragma unroll
-            for (int i = 0; i < 3; ++i) ys[i] = fma(A[j][i], sr, ys[i]);
+            for (int i = 0; i < 3; ++i) ys[i] = fma(sA[j][i][lane], sr, ys[i]);
           }
         }
       }
       double as[3];
-      accel_eval<PREC, G>(p, g, lane, lds, bc, t + T.c[s] * hs, ys, as);
-#pragma unroll
-      for (int j = 1; j < S; ++j)
-        if (j == s) {
-#pragma unroll
-          for (int i = 0; i < 3; ++i) A[j][i] = as[i];
-        }
+      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, stepping, bc, t + T.c[s] * hs, ys, as);
       if (stepping) ++evals;
     }
     if (!stepping) continue;
@@ -425,34 +430,34 @@ propagate_kernel(const PropParams p) {
       if (bj != 0.0) {
         const double hbj = hs * bj;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) yn[3 + i] = fma(A[j][i], hbj, yn[3 + i]);
+        for (int i = 0; i < 3; ++i) yn[3 + i] = fma(sA[j][i][lane], hbj, yn[3 + i]);
       }
       if (baj != 0.0) {
         const double hba = hh * baj;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) yn[i] = fma(A[j][i], hba, yn[i]);
+        for (int i = 0; i < 3; ++i) yn[i] = fma(sA[j][i][lane], hba, yn[i]);
       }
       if (ej != 0.0) {
         const double hej = hs * ej;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) kahan(err[3 + i], comp[3 + i], hej * A[j][i]);
+        for (int i = 0; i < 3; ++i) kahan(err[3 + i], comp[3 + i], hej * sA[j][i][lane]);
       }
       if (eaj != 0.0) {
         const double hea = hh * eaj;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) kahan(err[i], comp[i], hea * A[j][i]);
+        for (int i = 0; i < 3; ++i) kahan(err[i], comp[i], hea * sA[j][i][lane]);
       }
       if (MT::E3) {
         const double e3j = T.e3[j], e3aj = SO.e3a[j];
         if (e3j != 0.0) {
           const double h3 = hs * e3j;
 #pragma unroll
-          for (int i = 0; i < 3; ++i) kahan(err3[3 + i], comp3[3 + i], h3 * A[j][i]);
+          for (int i = 0; i < 3; ++i) kahan(err3[3 + i], comp3[3 + i], h3 * sA[j][i][lane]);
         }
         if (e3aj != 0.0) {
           const double h3a = hh * e3aj;
 #pragma unroll
-          for (int i = 0; i < 3; ++i) kahan(err3[i], comp3[i], h3a * A[j][i]);
+          for (int i = 0; i < 3; ++i) kahan(err3[i], comp3[i], h3a * sA[j][i][lane]);
         }
       }
     }
@@ -503,9 +508,7 @@ propagate_kernel(const PropParams p) {
       } else {
         h = clampd(hs * 0.1, hmin, hmax);
       }
-      continue;
-    }
-    if (en <= thresh || hs <= hmin) {
+    } else if (en <= thresh || hs <= hmin) {
       double tn;
       if (lands) {
         tn = tseg;
@@ -517,8 +520,8 @@ propagate_kernel(const PropParams p) {
       ++steps;
       rej_run = 0;
       if (MT::FSAL) {
-#pragma unroll
-        for (int i = 0; i < 3; ++i) A[0][i] = A[S - 1][i];
+        if (g == 0)
+          for (int i = 0; i < 3; ++i) sA[0][i][lane] = sA[S - 1][i][lane];
         have_k1 = true;
       } else {
         have_k1 = false;
@@ -584,6 +587,8 @@ propagate_kernel(const PropParams p) {
         just_rej = true;
       }
     }
+    // publish the FSAL copy of sA[0] before the next step reads it
+    if (MT::FSAL && G > 1) __syncthreads();
   }
 }
 
