@@ -72,6 +72,100 @@ def cpu_baseline(w, field, rows: int, days: float, threads: int) -> dict:
     }
 
 
+def run_ragged(args) -> dict:
+    """Ragged workload: 262,144 trajectories, 60% LEO (alt U(300,2000) km),
+    25% MEO (a U(20,000, 30,000) km), 15% GEO (a = 42,164 km, e < 1e-3),
+    durations log-uniform 1 h .. 30 d, degree/order 20, RK8(7)13M rtol 1e-11,
+    gravity only.  One persistent launch under a HIP graph; launch shape from
+    the autotuner's pilot."""
+    import bqp_amd
+    from bqp_amd import inputs
+    from bqp_amd.autotune import autotune
+    from bqp_amd.device import DeviceArray, synchronize
+
+    n = args.rows if args.rows != 65536 else 262144
+    w = inputs.workload("w2", n=n)
+    field = bqp_amd.kaula(20)
+    y0 = DeviceArray.from_numpy(w["y0"])
+    t0 = DeviceArray.from_numpy(np.zeros(n))
+    tf = DeviceArray.from_numpy(w["tf"])
+    orbit_days = float(np.sum(w["tf"]) / 86400.0)
+    prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-11)
+    tune = autotune(prop, w["y0"], 0.0, w["tf"], use_cache=False, cache_path=None)
+    prop.set_launch(**{**tune["launch"], "use_graph": 1})
+    frhs = prop.flops_per_rhs
+    fstage = stage_flops("rk87")
+    yf = DeviceArray((n, 6))
+    status = DeviceArray((n,), np.int32)
+    for _ in range(max(args.warmup, 1)):  # the first call captures the graph
+        prop.propagate_batch_final_dev(y0, t0, tf, None, yf, status)
+    wall = kern = flops = 0.0
+    for _ in range(args.steps):
+        synchronize()
+        ts = time.perf_counter()
+        _, _, st = prop.propagate_batch_final_dev(y0, t0, tf, None, yf, status)
+        synchronize()
+        wall += time.perf_counter() - ts
+        kern += st["kernel_ms"]
+        flops += st["evals"] * frhs + (st["steps"] + st["rejects"]) * fstage
+    ms = wall / args.steps * 1e3
+    kms = kern / args.steps
+    ok = int(np.sum(status.numpy() == 0))
+    prop.close()
+    achieved = flops / args.steps / (kms * 1e-3) / 1e12
+    print(f"[bench] ragged: {orbit_days / (ms * 1e-3):.1f} orbit-days/s, {ms:.1f} ms/step, {achieved:.2f} TFLOP/s, "
+          f"ok {ok}/{n}, launch {tune['launch']}", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, len(os.sched_getaffinity(0)))
+        cpu = cpu_baseline_ragged(w, field, threads)
+    return {
+        "metric": "orbit-days/s, 262,144 mixed LEO/MEO/GEO, log-uniform 1 h-30 d, deg-20 (ragged batch)",
+        "value": orbit_days / (ms * 1e-3),
+        "unit": "orbit-days/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded mixed-orbit elements and durations (PCG64 seed 0), seeded Kaula degree-20 field",
+        "config": {"workload": f"{n} mixed orbits, durations log-uniform 1 h..30 d ({orbit_days:.0f} orbit-days), "
+                               "RK8(7)13M rtol 1e-11, degree/order 20, gravity only, one persistent launch under a "
+                               "HIP graph", "orbits": n, "degree": 20, "launch": tune["launch"],
+                   "autotune_trials": tune["trials"]},
+        "roofline": {"bound": "mfma", "pipe": "FP64 vector ALU", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel_ms": kms,
+                     "flops_per_rhs": frhs},
+        "cpu_baseline": cpu,
+    }
+
+
+def cpu_baseline_ragged(w, field, threads: int) -> dict:
+    from oracle import oracle as orc
+
+    f = orc.Field(field.cbar, field.sbar, field.nmax, field.mu, field.radius)
+    model = orc.make_model(f)
+    integ = orc.make_integ(method="rk87", rtol=1e-11, canonical=(field.radius, field.mu))
+    rate_total, parts = 0.0, []
+    n = w["y0"].shape[0]
+    for cls, frac in ((0, 0.60), (1, 0.25), (2, 0.15)):
+        idx = np.nonzero(w["orbit_class"] == cls)[0][: 4 * threads]
+        t0 = time.perf_counter()
+        _, st, _, _ = orc.propagate_batch(model, integ, w["y0"][idx], 0.0, w["tf"][idx], threads=threads)
+        wall = time.perf_counter() - t0
+        days = float(np.sum(w["tf"][idx]) / 86400.0)
+        # time per orbit-day of this class, weighted by the class's share of the workload's orbit-days
+        share = float(np.sum(w["tf"][w["orbit_class"] == cls]) / np.sum(w["tf"]))
+        rate_total += share * wall / days
+        parts.append(f"class {cls}: {idx.size} rows, {days:.1f} orbit-days in {wall:.1f} s")
+    return {"value": 1.0 / rate_total, "unit": "orbit-days/s", "cores": threads, "kind": "port",
+            "sample": "first 4x threads rows of each orbit class at full duration, weighted by each class's share "
+                      "of the workload's orbit-days; " + "; ".join(parts)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,12 +177,18 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0)
     ap.add_argument("--launch", default="", help='JSON launch shape, e.g. {"waves_per_group": 2}')
+    ap.add_argument("--workload", default="w1", choices=["w1", "w2"],
+                    help="w1: 65,536 LEO x 7 d deg-70 (the headline); w2: ragged 262,144 mixed orbits deg-20")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build's path runs on a single MI355X "
-              f"(--gpus {args.gpus} is not supported; trajectories would shard by index range, "
-              f"see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py measures one GPU: the propagation path runs on a single MI355X "
+              f"(--gpus {args.gpus} is not supported; trajectories would shard by cost-weighted "
+              f"index ranges, one process per GPU).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "w2":
+        print(json.dumps(run_ragged(args)))
+        return
 
     import bqp_amd
     from bqp_amd import inputs

@@ -31,9 +31,10 @@ from ._native import ROW_STATUS, BqpError
 from .elements import OMEGA_EARTH, R_EARTH
 from .field import Field, goce, kaula, load_icgem
 from .propagator import Propagator, tableau
+from .surrogate import Surrogate
 
 __all__ = [
-    "Propagator", "Field", "BqpError", "kaula", "goce", "load_icgem", "tableau", "inputs", "elements",
+    "Propagator", "Surrogate", "Field", "BqpError", "kaula", "goce", "load_icgem", "tableau", "inputs", "elements",
     "kep_to_eci", "eci_to_kep", "propagate", "propagate_final", "propagate_batch_final", "gmst",
     "ROW_STATUS",
 ]

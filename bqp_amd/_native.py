@@ -26,7 +26,8 @@ ROW_STATUS = {
 METHODS = {"dopri5": 0, "dop853": 1, "rk87": 2, "dopri87": 2, "rk8(7)13m": 2}
 CONTROLS = {"scaled": 0, "absolute": 1, "compat": 2}
 DRAGS = {"none": 0, "vallado": 1, "compat": 2}
-PRECISIONS = {"fp64": 0, "split": 1}
+PRECISIONS = {"fp64": 0, "split": 1, "surrogate": 2}
+MLP_FORMATS = {"fp32": 0, "bf16": 1, "fp8": 2}
 
 
 class Config(C.Structure):
@@ -68,6 +69,19 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class SurrogateStats(C.Structure):
+    _fields_ = [
+        ("net_evals", C.c_int64),
+        ("exact_evals", C.c_int64),
+        ("checks", C.c_int64),
+        ("check_failures", C.c_int64),
+        ("band_fallbacks", C.c_int64),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Launch(C.Structure):
     _fields_ = [
         ("waves_per_group", C.c_int32),
@@ -93,6 +107,17 @@ EXPORTS = [
     "bqp_propagate_batch_sampled",
     "bqp_get_tableau",
     "bqp_flops_per_rhs",
+    "bqp_surrogate_param_count",
+    "bqp_surrogate_init",
+    "bqp_surrogate_get_params",
+    "bqp_surrogate_set_params",
+    "bqp_surrogate_dataset_dev",
+    "bqp_surrogate_train_dev",
+    "bqp_surrogate_loss_grad_dev",
+    "bqp_surrogate_pack",
+    "bqp_surrogate_infer_dev",
+    "bqp_surrogate_set_check",
+    "bqp_surrogate_last_stats",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -134,5 +159,20 @@ def load(path: str | None = None):
     L.bqp_get_tableau.argtypes = [C.c_int32, C.POINTER(Tableau)]
     L.bqp_flops_per_rhs.restype = C.c_double
     L.bqp_flops_per_rhs.argtypes = [C.c_void_p]
+    _fp = C.POINTER(C.c_float)
+    vp = C.c_void_p
+    L.bqp_surrogate_param_count.restype = C.c_int64
+    L.bqp_surrogate_param_count.argtypes = []
+    L.bqp_surrogate_init.argtypes = [vp, C.c_int32, C.c_double, C.c_double, C.c_uint64]
+    L.bqp_surrogate_get_params.argtypes = [vp, _fp, _dp]
+    L.bqp_surrogate_set_params.argtypes = [vp, _fp, _dp]
+    L.bqp_surrogate_dataset_dev.argtypes = [vp, C.c_int64, C.c_uint64, vp, vp, vp]
+    L.bqp_surrogate_train_dev.argtypes = [vp, vp, vp, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_uint64,
+                                          _fp]
+    L.bqp_surrogate_loss_grad_dev.argtypes = [vp, vp, vp, C.c_int64, _dp, _fp]
+    L.bqp_surrogate_pack.argtypes = [vp, C.c_int32, vp, C.c_int64]
+    L.bqp_surrogate_infer_dev.argtypes = [vp, C.c_int32, vp, C.c_int64, vp, _dp]
+    L.bqp_surrogate_set_check.argtypes = [vp, C.c_int32, C.c_double]
+    L.bqp_surrogate_last_stats.argtypes = [vp, C.POINTER(SurrogateStats)]
     _lib = L
     return L

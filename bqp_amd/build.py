@@ -10,7 +10,7 @@ import subprocess
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "bqp.hip")
+SRCS = [os.path.join(HERE, "csrc", "bqp.hip"), os.path.join(HERE, "csrc", "mlp.hip")]
 OUT = os.path.join(HERE, "libbqp.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -27,15 +27,33 @@ FLAGS = [
 
 
 def build(verbose: bool = False) -> str:
-    """Compile libbqp.so (always recompiles); returns its path."""
+    """Compile libbqp.so (always recompiles); returns its path.  The
+    translation units compile in parallel, then one link."""
+    objs = [os.path.join(HERE, "csrc", os.path.basename(src) + ".o") for src in SRCS]
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    procs = [subprocess.Popen([HIPCC, *compile_flags, "-c", "-o", obj, src], stdout=subprocess.PIPE,
+                              stderr=subprocess.PIPE, text=True) for src, obj in zip(SRCS, objs)]
+    failed = []
+    for src, p in zip(SRCS, procs):
+        out, err = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out + err)
+            failed.append(src)
+        elif verbose:
+            sys.stderr.write(err)
+    if failed:
+        raise RuntimeError(f"hipcc failed building {' '.join(failed)}")
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, SRC]
-    res = subprocess.run(cmd, capture_output=True, text=True)
+    res = subprocess.run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs],
+                         capture_output=True, text=True)
+    for obj in objs:
+        try:
+            os.remove(obj)
+        except OSError:
+            pass
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed ({res.returncode}) building {SRC}")
-    if verbose:
-        sys.stderr.write(res.stderr)
+        raise RuntimeError(f"hipcc link failed ({res.returncode})")
     os.replace(tmp, OUT)
     return OUT
 

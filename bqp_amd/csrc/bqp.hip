@@ -16,7 +16,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/bqp.h"
+#include "bqp_internal.hpp"
 #include "bqp_kernels.hpp"
 
 using namespace bqp;
@@ -32,39 +32,14 @@ struct GraphKey {
   bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
 };
 
-struct bqp_ctx {
-  bqp_config cfg{};
-  bqp_launch launch{};
-  hipStream_t stream = nullptr;
-  int device = 0;
-  int cus = 0;
-  // field
-  int nmax = -1, mmax = -1, jmax = -1;
-  double mu = 0.0, radius = 0.0;
-  GravEntry* d_tab = nullptr;
-  GravEntryF* d_tabf = nullptr;
-  size_t n_entries = 0;
-  // scratch (grown on demand, freed by bqp_destroy)
-  unsigned int* d_queue = nullptr;
-  unsigned long long* d_totals = nullptr;
-  int* d_first = nullptr;
-  void* d_stage = nullptr;
-  size_t stage_bytes = 0;
-  void* d_sort = nullptr;
-  size_t sort_bytes = 0;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  GraphKey graph_key{};
-  int64_t graph_builds = 0;
-  std::string err;
-};
-
-namespace {
-
-int fail(bqp_ctx* c, int code, const std::string& msg) {
+int bqp_fail(bqp_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
 }
+
+namespace {
+
+int fail(bqp_ctx* c, int code, const std::string& msg) { return bqp_fail(c, code, msg); }
 
 #define HIP_TRY(ctx, expr)                                                          \
   do {                                                                              \
@@ -73,7 +48,11 @@ int fail(bqp_ctx* c, int code, const std::string& msg) {
       return fail(ctx, BQP_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-int ensure(bqp_ctx* c, void** ptr, size_t* have, size_t need) {
+int ensure(bqp_ctx* c, void** ptr, size_t* have, size_t need) { return bqp_ensure(c, ptr, have, need); }
+
+}  // namespace
+
+int bqp_ensure(bqp_ctx* c, void** ptr, size_t* have, size_t need) {
   if (*have >= need && *ptr) return BQP_OK;
   if (*ptr) {
     (void)hipStreamSynchronize(c->stream);
@@ -87,10 +66,12 @@ int ensure(bqp_ctx* c, void** ptr, size_t* have, size_t need) {
   return BQP_OK;
 }
 
+namespace {
+
 bool valid_config(const bqp_config& c, std::string& why) {
   if (c.method < 0 || c.method > 2) { why = "method must be 0 (dopri5), 1 (dop853) or 2 (rk87)"; return false; }
   if (c.control < 0 || c.control > 2) { why = "control must be 0, 1 or 2"; return false; }
-  if (c.precision < 0 || c.precision > 1) { why = "precision must be 0 (fp64) or 1 (split)"; return false; }
+  if (c.precision < 0 || c.precision > 2) { why = "precision must be 0 (fp64), 1 (split) or 2 (surrogate)"; return false; }
   if (c.drag < 0 || c.drag > 2) { why = "drag must be 0, 1 or 2"; return false; }
   if (c.control == BQP_CONTROL_SCALED &&
       !(c.rtol >= 0.0 && c.atol_pos >= 0.0 && c.atol_vel >= 0.0 && (c.rtol > 0 || (c.atol_pos > 0 && c.atol_vel > 0)))) {
@@ -203,7 +184,11 @@ prop_fn pick_g(int G) {
 
 template <int M>
 prop_fn pick_p(int P, int G) {
-  return P == 0 ? pick_g<M, 0>(G) : pick_g<M, 1>(G);
+  if (P == 0) return pick_g<M, 0>(G);
+  if (P == 1) return pick_g<M, 1>(G);
+  // learned surrogate (2: BF16 network, 3: FP8 network): one wave per group
+  if (G != 1) return nullptr;
+  return P == 2 ? propagate_kernel<M, 2, 1> : propagate_kernel<M, 3, 1>;
 }
 
 prop_fn pick_kernel(int method, int prec, int G) {
@@ -247,6 +232,7 @@ void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, prop_fn ker
     q.order = order;
   }
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
+  if (p.sur_totals) hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, p.sur_totals, 5);
   if (events) (void)hipEventRecord(c->ev0, c->stream);
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * G), 0, c->stream, q);
   if (events) (void)hipEventRecord(c->ev1, c->stream);
@@ -258,14 +244,33 @@ void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, prop_fn ker
                      (int)n, c->d_first);
 }
 
+// Algorithmic FLOPs of one gravity evaluation (recursion + summation, FMA = 2):
+// column 0 costs 16 + 11 (N+1), column j >= 1 costs 20 + 20 (N+1-j).
+double grav_flops(int N, int jmax) {
+  double f = 16.0 + 11.0 * (N + 1.0);
+  for (int j = 1; j <= jmax; ++j) f += 20.0 + 20.0 * (N + 1.0 - j);
+  return f;
+}
+
 // Runs one propagation over device buffers; fills per-row outputs.
 int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
   if (c->nmax < 0 || !c->d_tab) return fail(c, BQP_E_NO_FIELD, "no gravity field loaded (call bqp_load_field)");
   if (n <= 0) return BQP_OK;
   if (n > 0x7fffffffLL) return fail(c, BQP_E_INVALID, "n exceeds 2^31-1 rows");
   const bqp_config& cfg = c->cfg;
-  const int G = c->launch.waves_per_group > 0 ? c->launch.waves_per_group : default_g(c, n);
-  prop_fn kern = pick_kernel(cfg.method, cfg.precision, G);
+  const bool sur = cfg.precision == BQP_PRECISION_SURROGATE;
+  int kprec = cfg.precision;
+  if (sur) {
+    const bqp_mlp_state* s = c->mlp;
+    if (!s || !s->d_tab_low) return fail(c, BQP_E_INVALID, "surrogate precision needs bqp_surrogate_init");
+    if (s->packed_fmt != BQP_MLP_BF16 && s->packed_fmt != BQP_MLP_FP8)
+      return fail(c, BQP_E_INVALID, "surrogate precision needs packed weights (bqp_surrogate_pack)");
+    if (c->launch.waves_per_group > 1)
+      return fail(c, BQP_E_INVALID, "surrogate precision runs one wave per group (waves_per_group 0 or 1)");
+    kprec = s->packed_fmt == BQP_MLP_FP8 ? 3 : 2;
+  }
+  const int G = sur ? 1 : (c->launch.waves_per_group > 0 ? c->launch.waves_per_group : default_g(c, n));
+  prop_fn kern = pick_kernel(cfg.method, kprec, G);
   if (!kern) return fail(c, BQP_E_INVALID, "waves_per_group must be 1, 2, 4 or 8");
   if (cfg.precision == BQP_PRECISION_SPLIT && !c->d_tabf)
     return fail(c, BQP_E_NO_FIELD, "split precision needs the FP32 table (reload the field)");
@@ -297,6 +302,26 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
   p.h0 = cfg.h0;
   p.h_min = cfg.h_min;
   p.h_max = cfg.h_max;
+  if (sur) {
+    const bqp_mlp_state* s = c->mlp;
+    p.low = field_view(c);
+    p.low.tab = s->d_tab_low;
+    p.low.tabf = nullptr;
+    p.low.nmax = std::min(s->nlow, c->nmax);
+    p.low.jmax = s->low_jmax;
+    p.mlp = s->packed;
+    p.sur_r_lo = s->r_lo;
+    p.sur_r_hi = s->r_hi;
+    p.sur_thresh2 = s->threshold * s->threshold;
+    p.sur_check_every = s->check_every;
+    p.sur_totals = s->d_sur_totals;
+  } else {
+    p.low = FieldView{};
+    p.mlp = MlpPacked{};
+    p.sur_r_lo = p.sur_r_hi = p.sur_thresh2 = 0.0;
+    p.sur_check_every = 0;
+    p.sur_totals = nullptr;
+  }
 
   const int64_t groups = (n + 63) / 64;
   int blocks = c->launch.blocks;
@@ -318,7 +343,11 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
     key.blocks = blocks;
     key.sort = sort;
     key.tab = c->d_tab;
-    if (!c->graph_exec || !(key == c->graph_key)) {
+    if (!c->graph_key) {
+      c->graph_key = new (std::nothrow) GraphKey();
+      if (!c->graph_key) return fail(c, BQP_E_DEVICE, "out of host memory");
+    }
+    if (!c->graph_exec || !(key == *c->graph_key)) {
       if (c->graph_exec) {
         (void)hipGraphExecDestroy(c->graph_exec);
         c->graph_exec = nullptr;
@@ -334,7 +363,7 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
         c->graph_exec = nullptr;
         return fail(c, BQP_E_DEVICE, std::string("graph instantiate: ") + hipGetErrorString(ie));
       }
-      c->graph_key = key;
+      std::memcpy((void*)c->graph_key, &key, sizeof(key));  // padding included (the key compares bytes)
       ++c->graph_builds;
     }
     HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
@@ -360,6 +389,24 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
     stats->first_failed_row = (tot[4] > 0) ? first : -1;
     stats->kernel_ms = ms;
     stats->flops = bqp_flops_per_rhs(c) * (double)tot[1];
+  }
+  if (sur) {
+    unsigned long long st[5] = {0, 0, 0, 0, 0};
+    HIP_TRY(c, hipMemcpy(st, c->mlp->d_sur_totals, sizeof(st), hipMemcpyDeviceToHost));
+    bqp_surrogate_stats& L = c->mlp->last;
+    L.net_evals = (int64_t)st[0];
+    L.exact_evals = (int64_t)st[1];
+    L.checks = (int64_t)st[2];
+    L.check_failures = (int64_t)st[3];
+    L.band_fallbacks = (int64_t)st[4];
+    if (stats) {
+      // network evaluation: MLP (MFMA) + exact low-degree part + frame/drag;
+      // exact evaluation: low-degree part (always computed) + the full RHS
+      const double full = bqp_flops_per_rhs(c);
+      const double gfull = grav_flops(c->nmax, c->jmax);
+      const double low = grav_flops(std::min(c->mlp->nlow, c->nmax), c->mlp->low_jmax);
+      stats->flops = (double)st[0] * (MLP_FLOPS + low + (full - gfull)) + (double)st[1] * (full + low);
+    }
   }
   if (tot[4] > 0) {
     char buf[160];
@@ -453,10 +500,12 @@ int32_t bqp_destroy(bqp_ctx* c) {
   (void)hipFree(c->d_first);
   (void)hipFree(c->d_stage);
   (void)hipFree(c->d_sort);
+  bqp_mlp_release(c);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c->graph_key;
   delete c;
   return BQP_OK;
 }
@@ -504,11 +553,14 @@ int32_t bqp_load_field(bqp_ctx* c, int32_t nmax, int32_t mmax, const double* cba
                          (float)e.yv, (float)e.yw, (float)e.zv, (float)e.zw};
   }
   (void)hipStreamSynchronize(c->stream);
+  bqp_mlp_release(c);  // a surrogate belongs to the field it was set up for
   (void)hipFree(c->d_tab);
   (void)hipFree(c->d_tabf);
   c->d_tab = nullptr;
   c->d_tabf = nullptr;
   c->nmax = -1;
+  c->h_cbar.assign(cbar, cbar + nt);
+  c->h_sbar.assign(sbar, sbar + nt);
   // two zero entries of padding: the unrolled column loops load entries in pairs
   tab.push_back(GravEntry{});
   tab.push_back(GravEntry{});
@@ -695,11 +747,87 @@ int32_t bqp_get_tableau(int32_t method, bqp_tableau* out) {
 // counted from the column sweep above: setup 16, column 0 11 per entry,
 // columns j >= 1: 20 for the sectoral step + its contribution, 20 per
 // recursion entry; rotation 14; drag 30 when enabled.
+int32_t bqp_surrogate_dataset_dev(bqp_ctx* c, int64_t n, uint64_t seed, double* d_pos, float* d_feat,
+                                  float* d_targ) {
+  if (!c) return BQP_E_INVALID;
+  bqp_mlp_state* s = c->mlp;
+  if (!s || !s->d_tab_low) return fail(c, BQP_E_INVALID, "surrogate not initialised (bqp_surrogate_init)");
+  if (n < 0 || (n > 0 && (!d_feat || !d_targ))) return fail(c, BQP_E_INVALID, "bad buffers");
+  if (n == 0) return BQP_OK;
+  FieldView full = field_view(c);
+  FieldView low = full;
+  low.tab = s->d_tab_low;
+  low.tabf = nullptr;
+  low.nmax = std::min(s->nlow, c->nmax);
+  low.jmax = s->low_jmax;
+  const double rmid = 0.5 * (s->r_lo + s->r_hi), rhalf = 0.5 * (s->r_hi - s->r_lo);
+  hipLaunchKernelGGL(surrogate_data_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, full, low,
+                     (long long)n, (unsigned long long)seed, s->r_lo, s->r_hi, (float)rmid, (float)(1.0 / rhalf),
+                     1.0 / s->out_scale[0], 1.0 / s->out_scale[1], 1.0 / s->out_scale[2], d_pos, d_feat, d_targ);
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return BQP_OK;
+}
+
+int32_t bqp_surrogate_init(bqp_ctx* c, int32_t nlow, double r_lo, double r_hi, uint64_t seed) {
+  if (!c) return BQP_E_INVALID;
+  if (c->nmax < 0) return fail(c, BQP_E_NO_FIELD, "no gravity field loaded (call bqp_load_field)");
+  if (nlow < 1 || nlow > 8 || nlow >= c->nmax)
+    return fail(c, BQP_E_INVALID, "need 1 <= nlow <= 8 and nlow < field degree");
+  if (!(r_lo > 0.0) || !(r_hi > r_lo) || !std::isfinite(r_hi))
+    return fail(c, BQP_E_INVALID, "need 0 < r_lo < r_hi (finite)");
+  if (!c->mlp) {
+    c->mlp = new (std::nothrow) bqp_mlp_state();
+    if (!c->mlp) return fail(c, BQP_E_DEVICE, "out of host memory");
+  }
+  bqp_mlp_state* s = c->mlp;
+  s->nlow = nlow;
+  s->r_lo = r_lo;
+  s->r_hi = r_hi;
+  s->check_every = 100;
+  s->threshold = 1e-9;
+  std::vector<GravEntry> tab;
+  int jmax = 0;
+  pack_field(nlow, std::min(c->mmax, nlow), c->h_cbar.data(), c->h_sbar.data(), c->mu, c->radius, tab, jmax);
+  tab.push_back(GravEntry{});
+  tab.push_back(GravEntry{});
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(s->d_tab_low);
+  s->d_tab_low = nullptr;
+  HIP_TRY(c, hipMalloc(&s->d_tab_low, tab.size() * sizeof(GravEntry)));
+  HIP_TRY(c, hipMemcpy(s->d_tab_low, tab.data(), tab.size() * sizeof(GravEntry), hipMemcpyHostToDevice));
+  s->low_jmax = jmax;
+  int rc = bqp_mlp_alloc_init(c, seed);
+  if (rc) return rc;
+  // output scales: per-component RMS of the residual over a seeded pilot sample
+  for (int i = 0; i < 3; ++i) s->out_scale[i] = 1.0;
+  const int64_t npilot = 65536;
+  float* d_f = nullptr;
+  float* d_t = nullptr;
+  HIP_TRY(c, hipMalloc(&d_f, (size_t)npilot * MLP_IN * sizeof(float)));
+  if (hipMalloc(&d_t, (size_t)npilot * 3 * sizeof(float)) != hipSuccess) {
+    (void)hipFree(d_f);
+    return fail(c, BQP_E_DEVICE, "hipMalloc failed");
+  }
+  rc = bqp_surrogate_dataset_dev(c, npilot, mix64(seed ^ 0x9170ull), nullptr, d_f, d_t);
+  std::vector<float> t((size_t)npilot * 3);
+  if (rc == BQP_OK && hipMemcpy(t.data(), d_t, t.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(c, BQP_E_DEVICE, "copy of the pilot sample failed");
+  (void)hipFree(d_f);
+  (void)hipFree(d_t);
+  if (rc) return rc;
+  for (int k = 0; k < 3; ++k) {
+    double ss = 0.0;
+    for (int64_t i = 0; i < npilot; ++i) ss += (double)t[3 * i + k] * (double)t[3 * i + k];
+    const double rms = std::sqrt(ss / (double)npilot);
+    s->out_scale[k] = (rms > 0.0 && std::isfinite(rms)) ? rms : 1.0;
+  }
+  return BQP_OK;
+}
+
 double bqp_flops_per_rhs(const bqp_ctx* c) {
   if (!c || c->nmax < 0) return 0.0;
-  const double N = c->nmax;
-  double f = 16.0 + 11.0 * (N + 1.0);
-  for (int j = 1; j <= c->jmax; ++j) f += 20.0 + 20.0 * (N + 1.0 - j);
+  double f = grav_flops(c->nmax, c->jmax);
   f += 14.0;
   if (c->cfg.drag != BQP_DRAG_NONE) f += 30.0;
   return f;

@@ -29,6 +29,7 @@
 #pragma once
 
 #include "bqp_device.hpp"
+#include "mlp_device.hpp"
 
 #ifndef BQP_WAVES_PER_EU
 #define BQP_WAVES_PER_EU 4
@@ -36,7 +37,7 @@
 
 namespace bqp {
 
-__constant__ bqp_tableau c_tab[3] = {BQP_DOPRI5_TABLEAU, BQP_DOP853_TABLEAU, BQP_RK87_TABLEAU};
+static __constant__ bqp_tableau c_tab[3] = {BQP_DOPRI5_TABLEAU, BQP_DOP853_TABLEAU, BQP_RK87_TABLEAU};
 
 // Second-order rearrangement of each tableau, computed on the host
 // (make_second_order) and copied to the device when a context is created.
@@ -48,7 +49,7 @@ struct SecondOrderTableau {
   double e3a[BQP_MAX_STAGES];                   // (e3.A)_j
   double bsum, esum, e3sum;
 };
-__constant__ SecondOrderTableau c_so[3];
+static __constant__ SecondOrderTableau c_so[3];
 
 inline void make_second_order(const bqp_tableau& t, SecondOrderTableau& o) {
   const int S = t.stages;
@@ -133,6 +134,20 @@ struct PropParams {
   int max_steps;
   int max_rejects;
   double rtol, atol_pos, atol_vel, eps, h0, h_min, h_max;
+  // learned surrogate (PREC >= 2 only)
+  FieldView low;                  // exact part, degree <= nlow
+  MlpPacked mlp;                  // packed network
+  double sur_r_lo, sur_r_hi;      // radius band (m)
+  double sur_thresh2;             // squared check threshold (m^2/s^4)
+  int sur_check_every;            // 0: never check
+  unsigned long long* sur_totals; // net, exact, checks, failures, band
+};
+
+// Per-lane surrogate bookkeeping (PREC >= 2).
+struct SurLane {
+  bool exact = false;       // row switched to the exact field after a failed check
+  bool check_now = false;   // this evaluation is a sampled check
+  unsigned net = 0, exact_n = 0, checks = 0, fails = 0, band = 0;
 };
 
 __device__ __forceinline__ double clampd(double v, double lo, double hi) {
@@ -146,6 +161,79 @@ __device__ __forceinline__ int lane_rank(unsigned long long mask) {
                                    __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
 
+// Gravity of PREC 2 (BF16 network) / 3 (FP8 network): exact degree <= nlow
+// part plus the network residual, or the exact field (outside the band,
+// after a failed check, and on sampled checks).  One wave = the workgroup.
+template <int PREC>
+__device__ __forceinline__ void sur_gravity(const PropParams& p, int lane, uint4* sX, float* sO, SurLane& sl,
+                                            bool store, double xb, double yb, double zb, double& gx, double& gy,
+                                            double& gz) {
+  grav_fp64<1>(p.low, 0, xb, yb, zb, gx, gy, gz);
+  const double r = sqrt(fma(xb, xb, fma(yb, yb, zb * zb)));
+  const bool band = (r >= p.sur_r_lo) && (r <= p.sur_r_hi);
+  const bool use_net = store && band && !sl.exact;
+  const bool check = use_net && sl.check_now;
+  const bool need_exact = store && (!use_net || check);
+  double rx = 0.0, ry = 0.0, rz = 0.0;
+  if (wave_ballot(use_net) != 0ull) {
+    float f[MLP_IN];
+    mlp_features((float)xb, (float)yb, (float)zb, p.mlp.r_mid, p.mlp.r_inv_half, f);
+    sX[lane * 4 + 0] = make_uint4(pack_bf16x2(f[0], f[1]), pack_bf16x2(f[2], f[3]), pack_bf16x2(f[4], f[5]),
+                                  pack_bf16x2(f[6], f[7]));
+    sX[lane * 4 + 1] = make_uint4(pack_bf16x2(f[8], f[9]), pack_bf16x2(f[10], f[11]), pack_bf16x2(f[12], f[13]),
+                                  pack_bf16x2(f[14], f[15]));
+    sX[lane * 4 + 2] = make_uint4(0u, 0u, 0u, 0u);
+    sX[lane * 4 + 3] = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+    constexpr int FMT = (PREC == 3) ? MLP_FMT_FP8 : MLP_FMT_BF16;
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
+      const uint4 xin[1] = {sX[(16 * t + (lane & 15)) * 4 + (lane >> 4)]};
+      f32x4 o[1];
+      mlp_forward_wave<FMT, 1>(p.mlp, lane, xin, o);
+      if (lane < 16) {
+        sO[(16 * t + lane) * 3 + 0] = o[0][0];
+        sO[(16 * t + lane) * 3 + 1] = o[0][1];
+        sO[(16 * t + lane) * 3 + 2] = o[0][2];
+      }
+    }
+    __syncthreads();
+    rx = (double)(sO[lane * 3 + 0] * p.mlp.out_scale[0]);
+    ry = (double)(sO[lane * 3 + 1] * p.mlp.out_scale[1]);
+    rz = (double)(sO[lane * 3 + 2] * p.mlp.out_scale[2]);
+    __syncthreads();
+  }
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  if (wave_ballot(need_exact) != 0ull) grav_fp64<1>(p.field, 0, xb, yb, zb, fx, fy, fz);
+  if (!store) return;
+  if (use_net) {
+    const double nx = gx + rx, ny = gy + ry, nz = gz + rz;
+    if (check) {
+      ++sl.checks;
+      const double dx = nx - fx, dy = ny - fy, dz = nz - fz;
+      if (fma(dx, dx, fma(dy, dy, dz * dz)) > p.sur_thresh2) {
+        ++sl.fails;
+        sl.exact = true;
+      }
+      gx = fx;
+      gy = fy;
+      gz = fz;
+      ++sl.exact_n;
+    } else {
+      gx = nx;
+      gy = ny;
+      gz = nz;
+      ++sl.net;
+    }
+  } else {
+    if (!band) ++sl.band;
+    gx = fx;
+    gy = fy;
+    gz = fz;
+    ++sl.exact_n;
+  }
+}
+
 // Inertial Cowell RHS: uniform z-rotation to the body frame, gravity, rotate
 // back, exponential drag (LY/src/rhs.rs:1119-1163 compute_drag).  The
 // acceleration is computed by wave 0 only and, for lanes with `store`,
@@ -156,7 +244,7 @@ template <int PREC, int G, int S>
 __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
                                            double (*lds)[3][64], double (*sA)[3][64], int slot,
                                            bool store, double bc, double t, const double* y,
-                                           double* acc) {
+                                           double* acc, uint4* sX, float* sO, SurLane& sl) {
   const double th = fma(p.model.omega, t, p.model.theta0);
   double s, c;
   sincos(th, &s, &c);
@@ -166,8 +254,10 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
   double gx, gy, gz;
   if (PREC == 0)
     grav_fp64<G>(p.field, g, xb, yb, zb, gx, gy, gz);
-  else
+  else if (PREC == 1)
     grav_split<G>(p.field, g, xb, yb, zb, gx, gy, gz);
+  else
+    sur_gravity<PREC>(p, lane, sX, sO, sl, store, xb, yb, zb, gx, gy, gz);
   if (G > 1) {
     lds[g][0][lane] = gx;
     lds[g][1][lane] = gy;
@@ -230,6 +320,10 @@ propagate_kernel(const PropParams p) {
   const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   __shared__ double lds[G][3][64];
   __shared__ unsigned int lds_base;
+  // surrogate: bf16 features of the wave's 64 lanes (32 k-slots each) and outputs
+  __shared__ uint4 sX[PREC >= 2 ? 64 * 4 : 1];
+  __shared__ float sO[PREC >= 2 ? 64 * 3 : 1];
+  SurLane sl;
 
   const bool compat = (p.control == CONTROL_COMPAT);
   const bool sampled = (p.t_eval != nullptr);
@@ -292,6 +386,7 @@ propagate_kernel(const PropParams p) {
           tcomp = 0.0;
           eprev = 0.0;
           have_prev = just_rej = have_k1 = false;
+          sl.exact = false;
           bc = p.bc ? p.bc[row] : 0.0;
           bool ok = isfinite(bc);
           for (int i = 0; i < 6; ++i) {
@@ -361,7 +456,8 @@ propagate_kernel(const PropParams p) {
       const bool needk = stepping && !have_k1;
       if (wave_ballot(needk) != 0ull) {
         double a0[3];
-        accel_eval<PREC, G, S>(p, g, lane, lds, sA, 0, needk, bc, t, y, a0);
+        if (PREC >= 2) sl.check_now = p.sur_check_every > 0 && (evals % p.sur_check_every) == 0;
+        accel_eval<PREC, G, S>(p, g, lane, lds, sA, 0, needk, bc, t, y, a0, sX, sO, sl);
         if (needk) {
           ++evals;
           have_k1 = true;
@@ -397,7 +493,8 @@ propagate_kernel(const PropParams p) {
         }
       }
       double as[3];
-      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, stepping, bc, t + T.c[s] * hs, ys, as);
+      if (PREC >= 2) sl.check_now = p.sur_check_every > 0 && (evals % p.sur_check_every) == 0;
+      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, stepping, bc, t + T.c[s] * hs, ys, as, sX, sO, sl);
       if (stepping) ++evals;
     }
     if (!stepping) continue;
@@ -590,6 +687,15 @@ propagate_kernel(const PropParams p) {
     // publish the FSAL copy of sA[0] before the next step reads it
     if (MT::FSAL && G > 1) __syncthreads();
   }
+  if (PREC >= 2) {
+    unsigned long long v[5] = {sl.net, sl.exact_n, sl.checks, sl.fails, sl.band};
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+      if (lane == 0 && v[k] != 0ull) atomicAdd(p.sur_totals + k, v[k]);
+    }
+  }
 }
 
 // Standalone batched gravity: one body-fixed position per lane.
@@ -610,6 +716,51 @@ __global__ void __launch_bounds__(256) gravity_kernel(FieldView f, const double*
     acc[3 * i + 1] = ay;
     acc[3 * i + 2] = az;
   }
+}
+
+// Surrogate training data: seeded body-fixed positions (uniform direction:
+// normalised Gaussian triple; |r| uniform in [r_lo, r_hi]), their features,
+// and the residual a_full - a_low (exact FP64 recursion) times inv_scale.
+__global__ void __launch_bounds__(256) surrogate_data_kernel(FieldView full, FieldView low, long long n,
+                                                             unsigned long long seed, double r_lo, double r_hi,
+                                                             float r_mid, float r_inv_half, double is0, double is1,
+                                                             double is2, double* __restrict__ pos,
+                                                             float* __restrict__ feat, float* __restrict__ targ) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = i < n;
+  const unsigned long long idx = ok ? (unsigned long long)i : 0ull;
+  double u[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k)  // uniforms in (0, 1]
+    u[k] = ((double)(mix64(seed ^ mix64(5ull * idx + (unsigned long long)k)) >> 11) + 1.0) * 0x1.0p-53;
+  const double two_pi = 6.283185307179586;
+  const double ra = sqrt(-2.0 * log(u[0])), rb = sqrt(-2.0 * log(u[2]));
+  double nx = ra * cos(two_pi * u[1]), ny = ra * sin(two_pi * u[1]), nz = rb * cos(two_pi * u[3]);
+  double nn = sqrt(fma(nx, nx, fma(ny, ny, nz * nz)));
+  if (!(nn > 1e-300)) {
+    nx = 0.0;
+    ny = 0.0;
+    nz = 1.0;
+    nn = 1.0;
+  }
+  const double r = fma(r_hi - r_lo, u[4], r_lo);
+  const double x = r * (nx / nn), y = r * (ny / nn), z = r * (nz / nn);
+  double fx, fy, fz, lx, ly, lz;
+  grav_fp64<1>(full, 0, x, y, z, fx, fy, fz);
+  grav_fp64<1>(low, 0, x, y, z, lx, ly, lz);
+  if (!ok) return;
+  if (pos) {
+    pos[3 * i] = x;
+    pos[3 * i + 1] = y;
+    pos[3 * i + 2] = z;
+  }
+  float f[MLP_IN];
+  mlp_features((float)x, (float)y, (float)z, r_mid, r_inv_half, f);
+#pragma unroll
+  for (int k = 0; k < MLP_IN; ++k) feat[i * MLP_IN + k] = f[k];
+  targ[3 * i] = (float)((fx - lx) * is0);
+  targ[3 * i + 1] = (float)((fy - ly) * is1);
+  targ[3 * i + 2] = (float)((fz - lz) * is2);
 }
 
 // Queue reset without hipMemsetAsync (graph-capture safe).

@@ -84,6 +84,13 @@ class DeviceArray:
             raise _native.BqpError("hipMemcpy D2H failed")
         return out
 
+    def rows(self, start: int, count: int) -> "DeviceView":
+        """Non-owning view of rows [start, start+count) along the first axis."""
+        if start < 0 or count < 0 or start + count > self.shape[0]:
+            raise ValueError("row range out of bounds")
+        row_bytes = self.nbytes // self.shape[0] if self.shape[0] else 0
+        return DeviceView(self.ptr + start * row_bytes, (count,) + self.shape[1:], self.dtype, self)
+
     def free(self):
         if self._ptr.value:
             hip().hipFree(self._ptr)
@@ -94,3 +101,21 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class DeviceView:
+    """Rows of a DeviceArray (keeps the parent alive)."""
+
+    def __init__(self, ptr: int, shape, dtype, parent):
+        self._p, self.shape, self.dtype, self._parent = int(ptr), tuple(shape), np.dtype(dtype), parent
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+
+    def data_ptr(self) -> int:
+        return self._p
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=self.dtype)
+        if self.nbytes and hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self._p), self.nbytes,
+                                           D2H) != 0:
+            raise _native.BqpError("hipMemcpy D2H failed")
+        return out

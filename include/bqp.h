@@ -64,6 +64,8 @@ extern "C" {
 
 #define BQP_PRECISION_FP64 0   /* plain FP64 (the correctness oracle form) */
 #define BQP_PRECISION_SPLIT 1  /* degree <= 4 in FP64, higher degrees FP32 + compensated sums */
+#define BQP_PRECISION_SURROGATE 2 /* degree <= nlow in FP64, the rest from the learned network
+                                     (bqp_surrogate_*), exact-field fallback; waves_per_group 1 */
 
 typedef struct bqp_ctx bqp_ctx;
 
@@ -96,7 +98,7 @@ typedef struct bqp_stats {
   int64_t rows_failed;
   int64_t first_failed_row; /* -1 if none */
   double kernel_ms;    /* device time of the propagation kernel (HIP events) */
-  double flops;        /* algorithmic FLOPs of the call */
+  double flops;        /* algorithmic FLOPs of the call: evals x bqp_flops_per_rhs + steps x stage sums */
 } bqp_stats;
 
 /* Launch shape (tuning); zero fields mean "default".  Chosen by the
@@ -155,6 +157,81 @@ int32_t bqp_get_tableau(int32_t method, bqp_tableau* out);
 
 /* Algorithmic FLOPs of one RHS evaluation at the loaded field / config. */
 double bqp_flops_per_rhs(const bqp_ctx* ctx);
+
+/* ------------------------------------------------------------------------
+ * Learned gravity surrogate.  No counterpart in the reference (which always
+ * evaluates every harmonic term, SC/src/gravity.rs:2638); these calls exist
+ * for the learned form of the propagation path.
+ *
+ * Network: 16 features of the body-fixed position -> 4 x [256, SiLU] -> 3,
+ * predicting (a_field - a_{degree <= nlow}) / out_scale.  FP32 master
+ * parameters (flat: W_l row-major (out x in), then b_l, for l = 0..4) are
+ * trained on the device with Adam; inference runs on MFMA in BF16 or with
+ * the three 256 x 256 layers in FP8 E4M3 (per-layer scales), FP32 accumulate.
+ * ---------------------------------------------------------------------- */
+#define BQP_MLP_FP32 0  /* FP32 vector path (training forward), reference form */
+#define BQP_MLP_BF16 1
+#define BQP_MLP_FP8 2
+
+typedef struct bqp_surrogate_stats {
+  int64_t net_evals;       /* RHS evaluations that used the network */
+  int64_t exact_evals;     /* evaluations that used the exact field */
+  int64_t checks;          /* sampled network-vs-exact comparisons */
+  int64_t check_failures;  /* comparisons above the threshold (row switched to exact) */
+  int64_t band_fallbacks;  /* evaluations outside the radius band */
+} bqp_surrogate_stats;
+
+/* Number of FP32 parameters of the network (202,499). */
+int64_t bqp_surrogate_param_count(void);
+
+/* Set up the surrogate for the loaded field: exact part up to degree nlow
+ * (1..8), radius band [r_lo, r_hi] (m) for features and fallback, seeded
+ * He-uniform initial parameters, output scales = per-component RMS of the
+ * residual over a seeded pilot sample.  Resets the optimiser. */
+int32_t bqp_surrogate_init(bqp_ctx* ctx, int32_t nlow, double r_lo, double r_hi, uint64_t seed);
+
+/* Copy the FP32 parameters (and the 3 output scales) out / in. */
+int32_t bqp_surrogate_get_params(bqp_ctx* ctx, float* params, double* out_scale);
+int32_t bqp_surrogate_set_params(bqp_ctx* ctx, const float* params, const double* out_scale);
+
+/* Seeded training data on the device: n body-fixed positions (uniform
+ * directions, |r| uniform in [r_lo, r_hi]), their 16 features, and the
+ * scaled residual targets computed with the exact FP64 field.  d_pos
+ * (n x 3 f64) may be NULL. */
+int32_t bqp_surrogate_dataset_dev(bqp_ctx* ctx, int64_t n, uint64_t seed, double* d_pos, float* d_feat,
+                                  float* d_targ);
+
+/* `steps` Adam steps (lr, beta1 0.9, beta2 0.999, eps 1e-8) on mean squared
+ * error over minibatches of `batch` rows drawn (seeded, with replacement)
+ * from the n device rows.  loss (host, may be NULL) receives each step's
+ * minibatch loss. */
+int32_t bqp_surrogate_train_dev(bqp_ctx* ctx, const float* d_feat, const float* d_targ, int64_t n,
+                                int32_t steps, int32_t batch, double lr, uint64_t seed, float* loss);
+
+/* Mean squared error over all n rows and its gradient (host, param_count
+ * floats, may be NULL), FP32. */
+int32_t bqp_surrogate_loss_grad_dev(bqp_ctx* ctx, const float* d_feat, const float* d_targ, int64_t n,
+                                    double* loss, float* grad);
+
+/* Pack the current parameters for MFMA inference in `format` (BF16 or
+ * FP8); FP8 activation scales are calibrated on n_calib feature rows. */
+int32_t bqp_surrogate_pack(bqp_ctx* ctx, int32_t format, const float* d_calib_feat, int64_t n_calib);
+
+/* Network outputs (n x 3, scaled units) for n feature rows in `format`
+ * (FP32 = vector path on the master parameters; BF16 / FP8 = MFMA path on
+ * the packed weights).  kernel_ms (may be NULL) = device time. */
+int32_t bqp_surrogate_infer_dev(bqp_ctx* ctx, int32_t format, const float* d_feat, int64_t n, float* d_out,
+                                double* kernel_ms);
+
+/* In-integrator use (precision BQP_PRECISION_SURROGATE): every
+ * `check_every`-th evaluation of a row (0 = never) also evaluates the exact
+ * field; a difference above `threshold` (m/s^2) switches that row to the
+ * exact field for the rest of its arc.  Positions outside the radius band
+ * always use the exact field. */
+int32_t bqp_surrogate_set_check(bqp_ctx* ctx, int32_t check_every, double threshold);
+
+/* Counters of the last propagation call. */
+int32_t bqp_surrogate_last_stats(const bqp_ctx* ctx, bqp_surrogate_stats* out);
 
 #ifdef __cplusplus
 }
