@@ -8,13 +8,15 @@ Workload (one "step" = one pass of the hot path over the whole batch):
 Inputs are resident in HBM before the timed region; the timed region is the
 device call (cost-sort + persistent propagation kernel + row statistics).
 
-`value` is the plain FP64 form.  The split-precision form runs on the same
-inputs after it; its rate and its final-position error against FP64 are
-reported under "forms".  The CPU baseline is the project's FP64 C
+`value` is the plain FP64 form.  The split-precision form and the learned
+surrogate (trained on the device first, untimed; BF16 network, exact
+fallback on a failed sampled check) run on the same inputs after it; their
+rates and final-position errors against FP64 are reported under "forms".  The CPU baseline is the project's FP64 C
 restatement (oracle/) on a bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N]
-       [--days D] [--forms fp64,split] [--no-cpu]
+       [--days D] [--forms fp64,split,surrogate] [--no-cpu]
+       [--workload w1|w2|w3]   (w2: ragged batch; w3: surrogate study)
 """
 from __future__ import annotations
 
@@ -70,6 +72,34 @@ def cpu_baseline(w, field, rows: int, days: float, threads: int) -> dict:
         "sample": f"first {rows} orbits of the same workload for {days:g} day(s) each "
                   f"({ok}/{rows} ok), FP64 C restatement, OpenMP dynamic schedule, wall {wall:.1f} s",
     }
+
+
+MLP_FLOPS = 2.0 * (16 * 256 + 3 * 256 * 256 + 256 * 3)  # per network evaluation
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
+FP8_PEAK_TFLOPS = 5000.0   # MI355X dense FP8 MFMA
+
+
+def train_surrogate(prop, rows: int, steps: int, batch: int = 8192, seed: int = 0):
+    """Set up and train the propagator's surrogate on `rows` seeded device
+    samples of its own field (residual above degree 4, |r| in [6578, 8378]
+    km).  Returns (surrogate, features, targets, report)."""
+    import bqp_amd
+    from bqp_amd.device import synchronize
+
+    sur = bqp_amd.Surrogate(prop, nlow=4, r_lo=6578.0e3, r_hi=8378.0e3, seed=seed)
+    t = time.perf_counter()
+    feat, targ, _ = sur.dataset(rows, seed=seed + 1)
+    synchronize()
+    t_data = time.perf_counter() - t
+    t = time.perf_counter()
+    losses = sur.fit(feat, targ, steps=steps, batch=batch, lr_max=2e-3, lr_min=1e-5, chunk=500, seed=seed)
+    synchronize()
+    t_train = time.perf_counter() - t
+    rep = {"samples": rows, "data_s": t_data, "steps": steps, "batch": batch, "train_s": t_train,
+           "loss_first": float(np.mean(losses[:20])), "loss_last": float(np.mean(losses[-50:]))}
+    print(f"[bench] surrogate trained: {rows} samples in {t_data:.2f} s, {steps} x {batch} steps in "
+          f"{t_train:.1f} s, loss {rep['loss_first']:.3g} -> {rep['loss_last']:.3g}", file=sys.stderr, flush=True)
+    return sur, feat, targ, rep
 
 
 def run_ragged(args) -> dict:
@@ -143,6 +173,122 @@ def run_ragged(args) -> dict:
     }
 
 
+def run_surrogate_study(args) -> dict:
+    """The learned-surrogate form on the headline workload (65,536 LEO x 7 d,
+    degree 70, Vallado drag, RK8(7)13M rtol 1e-12): device training on 10M
+    residual samples, MFMA inference rates (BF16 and FP8) against their
+    rooflines, network error against the 1e-9 m/s^2 check bound, the checked
+    in-integrator form against plain FP64 on the full workload, and the
+    unchecked network (an ablation: no exact fallback) on a 1-hour arc."""
+    import bqp_amd
+    from bqp_amd import inputs
+    from bqp_amd.device import DeviceArray, synchronize
+
+    n = args.rows
+    days = args.days
+    w = inputs.workload("w1", n=n)
+    field = bqp_amd.kaula(70)
+    prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado")
+    sur, feat, targ, train_rep = train_surrogate(prop, args.sur_rows, args.sur_steps)
+    scale = sur.out_scale
+    vfeat, vtarg, _ = sur.dataset(1_000_000, seed=977)
+    T = vtarg.numpy()
+    res = np.linalg.norm(T * scale, axis=1)
+    infer = {}
+    for fmt in ("fp32", "bf16", "fp8"):
+        if fmt != "fp32":
+            sur.pack(fmt, calib=feat.rows(0, 65536))
+        o, _ = sur.infer(vfeat, fmt)
+        err = np.linalg.norm((o.numpy() - T) * scale, axis=1)
+        rec = {"err_rms_m_s2": float(np.sqrt(np.mean(err ** 2))), "err_median_m_s2": float(np.median(err)),
+               "err_max_m_s2": float(err.max()), "frac_within_1e-9": float(np.mean(err <= 1e-9))}
+        if fmt != "fp32":
+            _, ms = sur.infer(feat, fmt)
+            _, ms = sur.infer(feat, fmt)
+            tf_s = MLP_FLOPS * feat.shape[0] / (ms * 1e-3) / 1e12
+            peak = BF16_PEAK_TFLOPS if fmt == "bf16" else FP8_PEAK_TFLOPS
+            rec.update({"rows": int(feat.shape[0]), "kernel_ms": ms, "achieved_tflops": tf_s, "peak_tflops": peak,
+                        "frac": tf_s / peak})
+        infer[fmt] = rec
+        print(f"[bench] surrogate {fmt}: {rec}", file=sys.stderr, flush=True)
+    residual = {"rms_m_s2": float(np.sqrt(np.mean(res ** 2))), "max_m_s2": float(res.max())}
+
+    y0 = DeviceArray.from_numpy(w["y0"])
+    t0 = DeviceArray.from_numpy(np.zeros(n))
+    bc = DeviceArray.from_numpy(w["bc"])
+    fstage = stage_flops("rk87")
+
+    def run(span, warm):
+        tf = DeviceArray.from_numpy(np.full(n, span))
+        yf = DeviceArray((n, 6))
+        st_ = DeviceArray((n,), np.int32)
+        if warm:
+            prop.propagate_batch_final_dev(y0, t0, tf, bc, yf, st_)
+        synchronize()
+        ts = time.perf_counter()
+        _, _, st = prop.propagate_batch_final_dev(y0, t0, tf, bc, yf, st_)
+        synchronize()
+        wall = time.perf_counter() - ts
+        return yf.numpy(), st_.numpy(), wall, st
+
+    out_runs = {}
+    spans = {"full": days * 86400.0, "1h": 3600.0}
+    ref = {}
+    for key, span in spans.items():
+        prop.reconfigure(precision="fp64")
+        y, s, wall, st = run(span, warm=(key == "full"))
+        ref[key] = y
+        out_runs[f"fp64_{key}"] = {"orbit_days_s": n * span / 86400.0 / wall, "wall_s": wall, "evals": st["evals"],
+                                   "steps": st["steps"], "ok": int(np.sum(s == 0))}
+        print(f"[bench] fp64 {key}: {out_runs[f'fp64_{key}']}", file=sys.stderr, flush=True)
+    prop.reconfigure(precision="surrogate")
+    cases = [("bf16", args.sur_check, "full"), ("fp8", args.sur_check, "full"), ("bf16", 0, "1h"), ("fp8", 0, "1h")]
+    for fmt, ce, key in cases:
+        sur.pack(fmt, calib=feat.rows(0, 65536))
+        sur.set_check(ce, 1e-9)
+        y, s, wall, st = run(spans[key], warm=False)
+        d = np.nan_to_num(np.linalg.norm(y[:, :3] - ref[key][:, :3], axis=1), nan=np.inf)
+        base = out_runs[f"fp64_{key}"]
+        rec = {"orbit_days_s": n * spans[key] / 86400.0 / wall, "wall_s": wall,
+               "speedup_vs_fp64": base["wall_s"] / wall, "evals": st["evals"], "steps": st["steps"],
+               "evals_vs_fp64": st["evals"] / max(base["evals"], 1), "ok": int(np.sum(s == 0)),
+               "pos_err_median_m": float(np.median(d)), "pos_err_max_m": float(d.max()),
+               "stats": sur.last_stats(),
+               "mlp_tflops": sur.last_stats()["net_evals"] * MLP_FLOPS / (st["kernel_ms"] * 1e-3) / 1e12}
+        tag = f"surrogate_{fmt}_{'checked' if ce else 'unchecked'}_{key}"
+        out_runs[tag] = rec
+        print(f"[bench] {tag}: {rec}", file=sys.stderr, flush=True)
+    prop.close()
+    main_run = out_runs["surrogate_bf16_checked_full"]
+    achieved = infer["bf16"]["achieved_tflops"]
+    return {
+        "metric": METRIC,
+        "value": main_run["orbit_days_s"],
+        "unit": "orbit-days/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": main_run["wall_s"] * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 network / f64 state",
+        "data": "synthetic: seeded LEO elements (PCG64 seed 0), seeded Kaula degree-70 field, seeded residual samples",
+        "config": {"workload": f"{n} LEO orbits x {days:g} days, RK8(7)13M rtol 1e-12, degree/order 70, Vallado "
+                               f"drag; learned residual above degree 4 with exact fallback (check every "
+                               f"{args.sur_check} evaluations, bound 1e-9 m/s^2)",
+                   "form": "surrogate (checked)", "truth_degree": 70},
+        "roofline": {"bound": "mfma", "pipe": "BF16 MFMA (standalone network kernel, 10M rows)",
+                     "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None},
+        "cpu_baseline": None,
+        "training": train_rep,
+        "residual": residual,
+        "inference": infer,
+        "runs": out_runs,
+    }
+
+
 def cpu_baseline_ragged(w, field, threads: int) -> dict:
     from oracle import oracle as orc
 
@@ -152,18 +298,25 @@ def cpu_baseline_ragged(w, field, threads: int) -> dict:
     rate_total, parts = 0.0, []
     n = w["y0"].shape[0]
     for cls, frac in ((0, 0.60), (1, 0.25), (2, 0.15)):
-        idx = np.nonzero(w["orbit_class"] == cls)[0][: 4 * threads]
-        t0 = time.perf_counter()
-        _, st, _, _ = orc.propagate_batch(model, integ, w["y0"][idx], 0.0, w["tf"][idx], threads=threads)
-        wall = time.perf_counter() - t0
+        members = np.nonzero(w["orbit_class"] == cls)[0]
+        rows = 4 * threads
+        while True:  # grow the sample until it is a few seconds of CPU work
+            idx = members[:rows]
+            t0 = time.perf_counter()
+            _, st, _, _ = orc.propagate_batch(model, integ, w["y0"][idx], 0.0, w["tf"][idx], threads=threads)
+            wall = time.perf_counter() - t0
+            if wall >= 3.0 or rows >= members.size:
+                break
+            rows = min(members.size, int(rows * max(2.0, 3.5 / max(wall, 1e-3))))
         days = float(np.sum(w["tf"][idx]) / 86400.0)
         # time per orbit-day of this class, weighted by the class's share of the workload's orbit-days
         share = float(np.sum(w["tf"][w["orbit_class"] == cls]) / np.sum(w["tf"]))
         rate_total += share * wall / days
         parts.append(f"class {cls}: {idx.size} rows, {days:.1f} orbit-days in {wall:.1f} s")
     return {"value": 1.0 / rate_total, "unit": "orbit-days/s", "cores": threads, "kind": "port",
-            "sample": "first 4x threads rows of each orbit class at full duration, weighted by each class's share "
-                      "of the workload's orbit-days; " + "; ".join(parts)}
+            "sample": "first rows of each orbit class at full duration (at least 4 x threads rows, grown until the "
+                      "class takes >= 3 s), weighted by each class's share of the workload's orbit-days; "
+                      + "; ".join(parts)}
 
 
 def main():
@@ -173,12 +326,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rows", type=int, default=65536)
     ap.add_argument("--days", type=float, default=7.0)
-    ap.add_argument("--forms", default="fp64,split")
+    ap.add_argument("--forms", default="fp64,split,surrogate")
+    ap.add_argument("--sur-rows", type=int, default=10_000_000, help="surrogate training samples")
+    ap.add_argument("--sur-steps", type=int, default=3000, help="surrogate Adam steps (batch 8192)")
+    ap.add_argument("--sur-check", type=int, default=100, help="surrogate: check every k-th evaluation")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0)
     ap.add_argument("--launch", default="", help='JSON launch shape, e.g. {"waves_per_group": 2}')
-    ap.add_argument("--workload", default="w1", choices=["w1", "w2"],
-                    help="w1: 65,536 LEO x 7 d deg-70 (the headline); w2: ragged 262,144 mixed orbits deg-20")
+    ap.add_argument("--workload", default="w1", choices=["w1", "w2", "w3"],
+                    help="w1: 65,536 LEO x 7 d deg-70 (the headline); w2: ragged 262,144 mixed orbits deg-20; "
+                         "w3: the surrogate study (training, MFMA inference rooflines, fallback, ablation)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the propagation path runs on a single MI355X "
@@ -188,6 +345,9 @@ def main():
 
     if args.workload == "w2":
         print(json.dumps(run_ragged(args)))
+        return
+    if args.workload == "w3":
+        print(json.dumps(run_surrogate_study(args)))
         return
 
     import bqp_amd
@@ -209,16 +369,22 @@ def main():
     forms = [f for f in args.forms.split(",") if f]
     results = {}
     finals = {}
+    sur_report = None
     for form in forms:
-        prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado", precision=form)
+        prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado",
+                                  precision="surrogate" if form == "surrogate" else form)
         if args.launch:
             prop.set_launch(**json.loads(args.launch))
+        if form == "surrogate":
+            sur, _, _, sur_report = train_surrogate(prop, args.sur_rows, args.sur_steps)
+            sur.pack("bf16")
+            sur.set_check(args.sur_check, 1e-9)
         frhs = prop.flops_per_rhs
         yf = DeviceArray((n, 6))
         status = DeviceArray((n,), np.int32)
         steps_t = DeviceArray((n,), np.int32)
         evals_t = DeviceArray((n,), np.int32)
-        for _ in range(args.warmup):
+        for _ in range(args.warmup if form == forms[0] else 0):
             prop.propagate_batch_final_dev(y0, t0, tf, bc, yf, status, steps_t, evals_t)
         synchronize()
         wall = 0.0
@@ -231,7 +397,8 @@ def main():
             synchronize()
             wall += time.perf_counter() - ts
             kern_ms += st["kernel_ms"]
-            flops += st["evals"] * frhs + (st["steps"] + st["rejects"]) * fstage
+            flops += (st["flops"] if form == "surrogate" else st["evals"] * frhs) + \
+                (st["steps"] + st["rejects"]) * fstage
         ok = int(np.sum(status.numpy() == 0))
         ms = wall / args.steps * 1e3
         kms = kern_ms / args.steps
@@ -248,6 +415,9 @@ def main():
             "achieved_tflops": achieved,
             "launch": getattr(prop, "launch", None),
         }
+        if form == "surrogate":
+            results[form]["surrogate_stats"] = sur.last_stats()
+            results[form]["training"] = sur_report
         finals[form] = yf.numpy()
         prop.close()
         print(f"[bench] {form}: {results[form]['value']:.1f} orbit-days/s, {ms:.1f} ms/step, "
@@ -269,6 +439,11 @@ def main():
                 "achieved_tflops": results[form]["achieved_tflops"],
                 "kernel_ms": results[form]["kernel_ms"],
             }
+            if form == "surrogate":
+                forms_out[form].update({
+                    "network": "16 features -> 4 x 256 SiLU -> 3, BF16 MFMA, FP32 accumulate, residual above deg 4",
+                    "check_every": args.sur_check, "threshold_m_s2": 1e-9,
+                    "stats": results[form]["surrogate_stats"], "training": results[form]["training"]})
 
     cpu = None
     if not args.no_cpu:

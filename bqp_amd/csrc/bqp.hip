@@ -17,7 +17,8 @@
 #include <vector>
 
 #include "bqp_internal.hpp"
-#include "bqp_kernels.hpp"
+#include "bqp_aux.hpp"
+#include "prop_forms.hpp"
 
 using namespace bqp;
 
@@ -169,35 +170,10 @@ FieldView field_view(const bqp_ctx* c) {
 
 // ---------------------------------------------------------------- launch
 
-typedef void (*prop_fn)(const PropParams);
-
-template <int M, int P>
-prop_fn pick_g(int G) {
-  switch (G) {
-    case 1: return propagate_kernel<M, P, 1>;
-    case 2: return propagate_kernel<M, P, 2>;
-    case 4: return propagate_kernel<M, P, 4>;
-    case 8: return propagate_kernel<M, P, 8>;
-    default: return nullptr;
-  }
-}
-
-template <int M>
-prop_fn pick_p(int P, int G) {
-  if (P == 0) return pick_g<M, 0>(G);
-  if (P == 1) return pick_g<M, 1>(G);
-  // learned surrogate (2: BF16 network, 3: FP8 network): one wave per group
-  if (G != 1) return nullptr;
-  return P == 2 ? propagate_kernel<M, 2, 1> : propagate_kernel<M, 3, 1>;
-}
-
 prop_fn pick_kernel(int method, int prec, int G) {
-  switch (method) {
-    case BQP_METHOD_DOPRI5: return pick_p<BQP_METHOD_DOPRI5>(prec, G);
-    case BQP_METHOD_DOP853: return pick_p<BQP_METHOD_DOP853>(prec, G);
-    case BQP_METHOD_RK87: return pick_p<BQP_METHOD_RK87>(prec, G);
-    default: return nullptr;
-  }
+  if (prec == 0) return pick_fp64(method, G);
+  if (prec == 1) return pick_split(method, G);
+  return pick_sur(method, prec, G);
 }
 
 int default_g(const bqp_ctx* c, int64_t n) {
@@ -265,11 +241,9 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
     if (!s || !s->d_tab_low) return fail(c, BQP_E_INVALID, "surrogate precision needs bqp_surrogate_init");
     if (s->packed_fmt != BQP_MLP_BF16 && s->packed_fmt != BQP_MLP_FP8)
       return fail(c, BQP_E_INVALID, "surrogate precision needs packed weights (bqp_surrogate_pack)");
-    if (c->launch.waves_per_group > 1)
-      return fail(c, BQP_E_INVALID, "surrogate precision runs one wave per group (waves_per_group 0 or 1)");
     kprec = s->packed_fmt == BQP_MLP_FP8 ? 3 : 2;
   }
-  const int G = sur ? 1 : (c->launch.waves_per_group > 0 ? c->launch.waves_per_group : default_g(c, n));
+  const int G = c->launch.waves_per_group > 0 ? c->launch.waves_per_group : default_g(c, n);
   prop_fn kern = pick_kernel(cfg.method, kprec, G);
   if (!kern) return fail(c, BQP_E_INVALID, "waves_per_group must be 1, 2, 4 or 8");
   if (cfg.precision == BQP_PRECISION_SPLIT && !c->d_tabf)
@@ -481,7 +455,9 @@ int32_t bqp_create(const bqp_config* cfg, bqp_ctx** out) {
     for (int j = 0; j < SECT_MAX; ++j) sectf[j] = (float)sect[j];
     if (hipMemcpyToSymbol(HIP_SYMBOL(c_so), so, sizeof(so)) != hipSuccess ||
         hipMemcpyToSymbol(HIP_SYMBOL(c_sect), sect, sizeof(sect)) != hipSuccess ||
-        hipMemcpyToSymbol(HIP_SYMBOL(c_sectf), sectf, sizeof(sectf)) != hipSuccess) {
+        hipMemcpyToSymbol(HIP_SYMBOL(c_sectf), sectf, sizeof(sectf)) != hipSuccess ||
+        upload_fp64(so, sect, sectf) != hipSuccess || upload_split(so, sect, sectf) != hipSuccess ||
+        upload_sur(so, sect, sectf) != hipSuccess || upload_sur_fp8(so, sect, sectf) != hipSuccess) {
       bqp_destroy(c);
       return BQP_E_DEVICE;
     }

@@ -161,21 +161,15 @@ __device__ __forceinline__ int lane_rank(unsigned long long mask) {
                                    __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
 }
 
-// Gravity of PREC 2 (BF16 network) / 3 (FP8 network): exact degree <= nlow
-// part plus the network residual, or the exact field (outside the band,
-// after a failed check, and on sampled checks).  One wave = the workgroup.
-template <int PREC>
-__device__ __forceinline__ void sur_gravity(const PropParams& p, int lane, uint4* sX, float* sO, SurLane& sl,
-                                            bool store, double xb, double yb, double zb, double& gx, double& gy,
-                                            double& gz) {
-  grav_fp64<1>(p.low, 0, xb, yb, zb, gx, gy, gz);
-  const double r = sqrt(fma(xb, xb, fma(yb, yb, zb * zb)));
-  const bool band = (r >= p.sur_r_lo) && (r <= p.sur_r_hi);
-  const bool use_net = store && band && !sl.exact;
-  const bool check = use_net && sl.check_now;
-  const bool need_exact = store && (!use_net || check);
-  double rx = 0.0, ry = 0.0, rz = 0.0;
-  if (wave_ballot(use_net) != 0ull) {
+// Network residual of the 64 positions of a trajectory group (PREC 2: BF16,
+// PREC 3: FP8 hidden layers).  Wave 0 writes the bf16 features of its 64
+// lanes to LDS; the four 16-sample tiles are dealt to the group's waves
+// (wave g runs tiles g, g+G, ...; with G = 8 waves 4..7 idle); the outputs
+// come back through LDS to wave 0.  Called by all G waves (barriers).
+template <int PREC, int G>
+__device__ __forceinline__ void sur_residual(const PropParams& p, int g, int lane, uint4* sX, float* sO, double xb,
+                                             double yb, double zb, double& rx, double& ry, double& rz) {
+  if (g == 0) {
     float f[MLP_IN];
     mlp_features((float)xb, (float)yb, (float)zb, p.mlp.r_mid, p.mlp.r_inv_half, f);
     sX[lane * 4 + 0] = make_uint4(pack_bf16x2(f[0], f[1]), pack_bf16x2(f[2], f[3]), pack_bf16x2(f[4], f[5]),
@@ -184,41 +178,48 @@ __device__ __forceinline__ void sur_gravity(const PropParams& p, int lane, uint4
                                   pack_bf16x2(f[14], f[15]));
     sX[lane * 4 + 2] = make_uint4(0u, 0u, 0u, 0u);
     sX[lane * 4 + 3] = make_uint4(0u, 0u, 0u, 0u);
-    __syncthreads();
-    constexpr int FMT = (PREC == 3) ? MLP_FMT_FP8 : MLP_FMT_BF16;
+  }
+  __syncthreads();
+  constexpr int FMT = (PREC == 3) ? MLP_FMT_FP8 : MLP_FMT_BF16;
 #pragma unroll 1
-    for (int t = 0; t < 4; ++t) {
-      const uint4 xin[1] = {sX[(16 * t + (lane & 15)) * 4 + (lane >> 4)]};
-      f32x4 o[1];
-      mlp_forward_wave<FMT, 1>(p.mlp, lane, xin, o);
-      if (lane < 16) {
-        sO[(16 * t + lane) * 3 + 0] = o[0][0];
-        sO[(16 * t + lane) * 3 + 1] = o[0][1];
-        sO[(16 * t + lane) * 3 + 2] = o[0][2];
-      }
+  for (int t = g; t < 4; t += G) {
+    const uint4 xin[1] = {sX[(16 * t + (lane & 15)) * 4 + (lane >> 4)]};
+    f32x4 o[1];
+    mlp_forward_wave<FMT, 1>(p.mlp, lane, xin, o);
+    if (lane < 16) {
+      sO[(16 * t + lane) * 3 + 0] = o[0][0];
+      sO[(16 * t + lane) * 3 + 1] = o[0][1];
+      sO[(16 * t + lane) * 3 + 2] = o[0][2];
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  if (g == 0) {
     rx = (double)(sO[lane * 3 + 0] * p.mlp.out_scale[0]);
     ry = (double)(sO[lane * 3 + 1] * p.mlp.out_scale[1]);
     rz = (double)(sO[lane * 3 + 2] * p.mlp.out_scale[2]);
-    __syncthreads();
   }
-  double fx = 0.0, fy = 0.0, fz = 0.0;
-  if (wave_ballot(need_exact) != 0ull) grav_fp64<1>(p.field, 0, xb, yb, zb, fx, fy, fz);
+}
+
+// Wave 0 of a surrogate group: exact low-degree part + residual, or the
+// exact field (gx, gy, gz on entry: the full FP64 field where needed).  A
+// sampled check whose difference exceeds the threshold moves the row to the
+// exact field for the rest of its arc.
+__device__ __forceinline__ void sur_combine(const PropParams& p, SurLane& sl, bool store, bool band, bool use_net,
+                                            bool check, double xb, double yb, double zb, double rx, double ry,
+                                            double rz, double& gx, double& gy, double& gz) {
+  double lx, ly, lz;
+  grav_fp64<1>(p.low, 0, xb, yb, zb, lx, ly, lz);
   if (!store) return;
   if (use_net) {
-    const double nx = gx + rx, ny = gy + ry, nz = gz + rz;
+    const double nx = lx + rx, ny = ly + ry, nz = lz + rz;
     if (check) {
       ++sl.checks;
-      const double dx = nx - fx, dy = ny - fy, dz = nz - fz;
+      const double dx = nx - gx, dy = ny - gy, dz = nz - gz;
       if (fma(dx, dx, fma(dy, dy, dz * dz)) > p.sur_thresh2) {
         ++sl.fails;
         sl.exact = true;
       }
-      gx = fx;
-      gy = fy;
-      gz = fz;
-      ++sl.exact_n;
+      ++sl.exact_n;  // the exact value, already computed, is used
     } else {
       gx = nx;
       gy = ny;
@@ -227,9 +228,6 @@ __device__ __forceinline__ void sur_gravity(const PropParams& p, int lane, uint4
     }
   } else {
     if (!band) ++sl.band;
-    gx = fx;
-    gy = fy;
-    gz = fz;
     ++sl.exact_n;
   }
 }
@@ -244,7 +242,7 @@ template <int PREC, int G, int S>
 __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
                                            double (*lds)[3][64], double (*sA)[3][64], int slot,
                                            bool store, double bc, double t, const double* y,
-                                           double* acc, uint4* sX, float* sO, SurLane& sl) {
+                                           double* acc, uint4* sX, float* sO, int* sFlag, SurLane& sl) {
   const double th = fma(p.model.omega, t, p.model.theta0);
   double s, c;
   sincos(th, &s, &c);
@@ -252,12 +250,23 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
   const double yb = fma(c, y[1], -(s * y[0]));
   const double zb = y[2];
   double gx, gy, gz;
-  if (PREC == 0)
+  // surrogate decisions (identical in every wave of the group)
+  bool band = true, use_net = false, check = false;
+  double rx = 0.0, ry = 0.0, rz = 0.0;
+  if (PREC == 0) {
     grav_fp64<G>(p.field, g, xb, yb, zb, gx, gy, gz);
-  else if (PREC == 1)
+  } else if (PREC == 1) {
     grav_split<G>(p.field, g, xb, yb, zb, gx, gy, gz);
-  else
-    sur_gravity<PREC>(p, lane, sX, sO, sl, store, xb, yb, zb, gx, gy, gz);
+  } else {
+    const double r = sqrt(fma(xb, xb, fma(yb, yb, zb * zb)));
+    band = (r >= p.sur_r_lo) && (r <= p.sur_r_hi);
+    use_net = store && band && !sl.exact;
+    check = use_net && sl.check_now;
+    const bool need_exact = store && (!use_net || check);
+    if (wave_ballot(use_net) != 0ull) sur_residual<PREC, G>(p, g, lane, sX, sO, xb, yb, zb, rx, ry, rz);
+    gx = gy = gz = 0.0;
+    if (wave_ballot(need_exact) != 0ull) grav_fp64<G>(p.field, g, xb, yb, zb, gx, gy, gz);
+  }
   if (G > 1) {
     lds[g][0][lane] = gx;
     lds[g][1][lane] = gy;
@@ -277,7 +286,12 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
   // would be identical, so they skip the inertial rotation and the drag
   if (G > 1 && g != 0) {
     __syncthreads();
+    if (PREC >= 2) sl.exact = sFlag[lane] != 0;  // wave 0's verdict of this evaluation's check
     return;
+  }
+  if (PREC >= 2) {
+    sur_combine(p, sl, store, band, use_net, check, xb, yb, zb, rx, ry, rz, gx, gy, gz);
+    if (G > 1) sFlag[lane] = sl.exact ? 1 : 0;
   }
   double ax = fma(c, gx, -(s * gy));
   double ay = fma(s, gx, c * gy);
@@ -323,6 +337,7 @@ propagate_kernel(const PropParams p) {
   // surrogate: bf16 features of the wave's 64 lanes (32 k-slots each) and outputs
   __shared__ uint4 sX[PREC >= 2 ? 64 * 4 : 1];
   __shared__ float sO[PREC >= 2 ? 64 * 3 : 1];
+  __shared__ int sFlag[PREC >= 2 && G > 1 ? 64 : 1];
   SurLane sl;
 
   const bool compat = (p.control == CONTROL_COMPAT);
@@ -457,7 +472,7 @@ propagate_kernel(const PropParams p) {
       if (wave_ballot(needk) != 0ull) {
         double a0[3];
         if (PREC >= 2) sl.check_now = p.sur_check_every > 0 && (evals % p.sur_check_every) == 0;
-        accel_eval<PREC, G, S>(p, g, lane, lds, sA, 0, needk, bc, t, y, a0, sX, sO, sl);
+        accel_eval<PREC, G, S>(p, g, lane, lds, sA, 0, needk, bc, t, y, a0, sX, sO, sFlag, sl);
         if (needk) {
           ++evals;
           have_k1 = true;
@@ -494,7 +509,7 @@ propagate_kernel(const PropParams p) {
       }
       double as[3];
       if (PREC >= 2) sl.check_now = p.sur_check_every > 0 && (evals % p.sur_check_every) == 0;
-      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, stepping, bc, t + T.c[s] * hs, ys, as, sX, sO, sl);
+      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, stepping, bc, t + T.c[s] * hs, ys, as, sX, sO, sFlag, sl);
       if (stepping) ++evals;
     }
     if (!stepping) continue;
@@ -687,7 +702,7 @@ propagate_kernel(const PropParams p) {
     // publish the FSAL copy of sA[0] before the next step reads it
     if (MT::FSAL && G > 1) __syncthreads();
   }
-  if (PREC >= 2) {
+  if (PREC >= 2 && g == 0) {
     unsigned long long v[5] = {sl.net, sl.exact_n, sl.checks, sl.fails, sl.band};
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -696,180 +711,6 @@ propagate_kernel(const PropParams p) {
       if (lane == 0 && v[k] != 0ull) atomicAdd(p.sur_totals + k, v[k]);
     }
   }
-}
-
-// Standalone batched gravity: one body-fixed position per lane.
-template <int PREC>
-__global__ void __launch_bounds__(256) gravity_kernel(FieldView f, const double* __restrict__ pos,
-                                                      double* __restrict__ acc, long long n) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool ok = i < n;
-  const long long r = ok ? i : 0;
-  const double x = pos[3 * r], y = pos[3 * r + 1], z = pos[3 * r + 2];
-  double ax, ay, az;
-  if (PREC == 0)
-    grav_fp64<1>(f, 0, x, y, z, ax, ay, az);
-  else
-    grav_split<1>(f, 0, x, y, z, ax, ay, az);
-  if (ok) {
-    acc[3 * i] = ax;
-    acc[3 * i + 1] = ay;
-    acc[3 * i + 2] = az;
-  }
-}
-
-// Surrogate training data: seeded body-fixed positions (uniform direction:
-// normalised Gaussian triple; |r| uniform in [r_lo, r_hi]), their features,
-// and the residual a_full - a_low (exact FP64 recursion) times inv_scale.
-__global__ void __launch_bounds__(256) surrogate_data_kernel(FieldView full, FieldView low, long long n,
-                                                             unsigned long long seed, double r_lo, double r_hi,
-                                                             float r_mid, float r_inv_half, double is0, double is1,
-                                                             double is2, double* __restrict__ pos,
-                                                             float* __restrict__ feat, float* __restrict__ targ) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool ok = i < n;
-  const unsigned long long idx = ok ? (unsigned long long)i : 0ull;
-  double u[5];
-#pragma unroll
-  for (int k = 0; k < 5; ++k)  // uniforms in (0, 1]
-    u[k] = ((double)(mix64(seed ^ mix64(5ull * idx + (unsigned long long)k)) >> 11) + 1.0) * 0x1.0p-53;
-  const double two_pi = 6.283185307179586;
-  const double ra = sqrt(-2.0 * log(u[0])), rb = sqrt(-2.0 * log(u[2]));
-  double nx = ra * cos(two_pi * u[1]), ny = ra * sin(two_pi * u[1]), nz = rb * cos(two_pi * u[3]);
-  double nn = sqrt(fma(nx, nx, fma(ny, ny, nz * nz)));
-  if (!(nn > 1e-300)) {
-    nx = 0.0;
-    ny = 0.0;
-    nz = 1.0;
-    nn = 1.0;
-  }
-  const double r = fma(r_hi - r_lo, u[4], r_lo);
-  const double x = r * (nx / nn), y = r * (ny / nn), z = r * (nz / nn);
-  double fx, fy, fz, lx, ly, lz;
-  grav_fp64<1>(full, 0, x, y, z, fx, fy, fz);
-  grav_fp64<1>(low, 0, x, y, z, lx, ly, lz);
-  if (!ok) return;
-  if (pos) {
-    pos[3 * i] = x;
-    pos[3 * i + 1] = y;
-    pos[3 * i + 2] = z;
-  }
-  float f[MLP_IN];
-  mlp_features((float)x, (float)y, (float)z, r_mid, r_inv_half, f);
-#pragma unroll
-  for (int k = 0; k < MLP_IN; ++k) feat[i * MLP_IN + k] = f[k];
-  targ[3 * i] = (float)((fx - lx) * is0);
-  targ[3 * i + 1] = (float)((fy - ly) * is1);
-  targ[3 * i + 2] = (float)((fz - lz) * is2);
-}
-
-// Queue reset without hipMemsetAsync (graph-capture safe).
-__global__ void reset_counter_kernel(unsigned int* q) { q[0] = 0u; }
-
-// Longest-first hand-out order (a counting sort made of plain kernels, so it
-// can be captured in a HIP graph).  The cost key is the number of orbit
-// revolutions over the span (span * mean motion from vis-viva); at a fixed
-// tolerance the step count per revolution varies slowly, so rows with similar
-// keys take similar step counts and share a wave.  Row results do not depend
-// on which lane runs them, so any tie order is exact.
-constexpr int SORT_BUCKETS = 4096;
-
-__device__ __forceinline__ int cost_bucket(const double* __restrict__ s, double span, double mu) {
-  const double r = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
-  const double v2 = s[3] * s[3] + s[4] * s[4] + s[5] * s[5];
-  const double en = 0.5 * v2 - mu / r;
-  double rev = 0.0;
-  if (en < 0.0 && r > 0.0 && span > 0.0) {
-    const double a = -mu / (2.0 * en);
-    rev = span * sqrt(mu / (a * a * a));
-  }
-  // 128 buckets per octave of revolutions, most expensive first
-  int b = isfinite(rev) && rev > 0.0 ? (int)(log2(1.0 + rev) * 128.0) : 0;
-  b = b < 0 ? 0 : (b > SORT_BUCKETS - 1 ? SORT_BUCKETS - 1 : b);
-  return SORT_BUCKETS - 1 - b;
-}
-
-__global__ void sort_clear_kernel(unsigned int* __restrict__ hist) {
-  for (int i = threadIdx.x; i < SORT_BUCKETS; i += blockDim.x) hist[i] = 0u;
-}
-
-__global__ void sort_hist_kernel(const double* __restrict__ y0, const double* __restrict__ t0,
-                                 const double* __restrict__ tf, double mu, int n,
-                                 unsigned int* __restrict__ hist, unsigned short* __restrict__ bucket) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int b = cost_bucket(y0 + 6 * (size_t)i, tf[i] - t0[i], mu);
-  bucket[i] = (unsigned short)b;
-  atomicAdd(&hist[b], 1u);
-}
-
-// exclusive scan of the histogram in place (one block of 1024 threads)
-__global__ void __launch_bounds__(1024) sort_scan_kernel(unsigned int* __restrict__ hist) {
-  __shared__ unsigned int part[1024];
-  constexpr int PER = SORT_BUCKETS / 1024;
-  const int t = threadIdx.x;
-  unsigned int loc[PER], sum = 0;
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    loc[k] = hist[t * PER + k];
-    sum += loc[k];
-  }
-  part[t] = sum;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const unsigned int v = t >= off ? part[t - off] : 0u;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  unsigned int run = part[t] - sum;
-#pragma unroll
-  for (int k = 0; k < PER; ++k) {
-    hist[t * PER + k] = run;
-    run += loc[k];
-  }
-}
-
-__global__ void sort_scatter_kernel(const unsigned short* __restrict__ bucket, int n,
-                                    unsigned int* __restrict__ offs, int32_t* __restrict__ order) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const unsigned int pos = atomicAdd(&offs[bucket[i]], 1u);
-  order[pos] = i;
-}
-
-// Sum of per-row counters into 64-bit totals (stats).
-__global__ void row_stats_kernel(const int32_t* __restrict__ status, const int32_t* __restrict__ steps,
-                                 const int32_t* __restrict__ evals, const int32_t* __restrict__ rejects,
-                                 int n, unsigned long long* __restrict__ out) {
-  __shared__ unsigned long long acc[5];
-  if (threadIdx.x < 5) acc[threadIdx.x] = 0ull;
-  __syncthreads();
-  unsigned long long s = 0, e = 0, r = 0, ok = 0, bad = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    s += (unsigned)steps[i];
-    e += (unsigned)evals[i];
-    r += (unsigned)rejects[i];
-    if (status[i] == 0) ++ok; else ++bad;
-  }
-  atomicAdd(&acc[0], s);
-  atomicAdd(&acc[1], e);
-  atomicAdd(&acc[2], r);
-  atomicAdd(&acc[3], ok);
-  atomicAdd(&acc[4], bad);
-  __syncthreads();
-  if (threadIdx.x < 5) atomicAdd(&out[threadIdx.x], acc[threadIdx.x]);
-}
-
-__global__ void first_failed_kernel(const int32_t* __restrict__ status, int n, int* __restrict__ out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && status[i] != 0) atomicMin(out, i);
-}
-
-__global__ void fill_int_kernel(int* p, int v) { p[0] = v; }
-__global__ void zero_u64_kernel(unsigned long long* p, int n) {
-  const int i = threadIdx.x;
-  if (i < n) p[i] = 0ull;
 }
 
 }  // namespace bqp

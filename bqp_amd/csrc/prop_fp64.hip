@@ -1,0 +1,16 @@
+// Plain FP64 form of the persistent propagation kernel (the correctness
+// reference of the other forms).
+#include "prop_forms.hpp"
+
+namespace bqp {
+BQP_FORM_UNIT(pick_fp64, upload_fp64, 0)
+
+prop_fn pick_fp64(int method, int G) {
+  switch (method) {
+    case BQP_METHOD_DOPRI5: return pick_w<BQP_METHOD_DOPRI5, 0>(G);
+    case BQP_METHOD_DOP853: return pick_w<BQP_METHOD_DOP853, 0>(G);
+    case BQP_METHOD_RK87: return pick_w<BQP_METHOD_RK87, 0>(G);
+    default: return nullptr;
+  }
+}
+}  // namespace bqp

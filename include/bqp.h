@@ -65,7 +65,7 @@ extern "C" {
 #define BQP_PRECISION_FP64 0   /* plain FP64 (the correctness oracle form) */
 #define BQP_PRECISION_SPLIT 1  /* degree <= 4 in FP64, higher degrees FP32 + compensated sums */
 #define BQP_PRECISION_SURROGATE 2 /* degree <= nlow in FP64, the rest from the learned network
-                                     (bqp_surrogate_*), exact-field fallback; waves_per_group 1 */
+                                     (bqp_surrogate_*), exact-field fallback */
 
 typedef struct bqp_ctx bqp_ctx;
 

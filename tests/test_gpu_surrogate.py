@@ -149,13 +149,14 @@ def _leo(n, field_n, span):
     return w["y0"], np.zeros(n), np.full(n, span)
 
 
-def test_failed_check_falls_back_bit_identically(field20):
-    announce("surrogate fallback: threshold 0 -> every row exact; vs FP64 G=1 bitwise")
+@pytest.mark.parametrize("G", [1, 2, 4, 8])
+def test_failed_check_falls_back_bit_identically(field20, G):
+    announce(f"surrogate fallback G={G}: threshold 0 -> every row exact; vs FP64 bitwise")
     y0, t0, tf = _leo(64, 20, 7200.0)
-    with bqp_amd.Propagator(field20, method="rk87", rtol=1e-10, launch={"waves_per_group": 1}) as p:
+    with bqp_amd.Propagator(field20, method="rk87", rtol=1e-10, launch={"waves_per_group": G}) as p:
         yref, sref, iref = p.propagate_batch_final(y0, t0, tf)
-    prop, sur = _surrogate(field20, seed=2, precision="surrogate")
-    sur.pack("bf16")
+    prop, sur = _surrogate(field20, seed=2, precision="surrogate", launch={"waves_per_group": G})
+    sur.pack("bf16" if G != 2 else "fp8", calib=sur.dataset(4096, seed=5)[0])
     sur.set_check(1, 0.0)
     y, s, info = prop.propagate_batch_final(y0, t0, tf)
     st = sur.last_stats()
@@ -170,7 +171,8 @@ def test_outside_band_uses_exact_field(field20):
     y0, t0, tf = _leo(64, 20, 3600.0)
     with bqp_amd.Propagator(field20, method="rk87", rtol=1e-10, launch={"waves_per_group": 1}) as p:
         yref, _, _ = p.propagate_batch_final(y0, t0, tf)
-    prop = bqp_amd.Propagator(field20, method="rk87", rtol=1e-10, precision="surrogate")
+    prop = bqp_amd.Propagator(field20, method="rk87", rtol=1e-10, precision="surrogate",
+                              launch={"waves_per_group": 1})
     sur = bqp_amd.Surrogate(prop, nlow=4, r_lo=20000e3, r_hi=30000e3, seed=2)
     sur.pack("bf16")
     sur.set_check(0, 1e-9)
@@ -203,3 +205,12 @@ def test_network_beats_truncated_field(trained, field20):
     announce(f"median position error after 6000 s: network {np.median(e_net):.3g} m, "
              f"truncated field {np.median(e_tr):.3g} m")
     assert np.median(e_net) < 0.5 * np.median(e_tr)
+    # the residual network split over the waves of a group gives the same
+    # network values, so the same trajectories
+    prop.reconfigure(precision="surrogate")
+    prop.set_launch(waves_per_group=4)
+    y4, s4, info4 = prop.propagate_batch_final(y0, t0, tf)
+    prop.set_launch(waves_per_group=0)
+    prop.reconfigure(precision="fp64")
+    assert np.all(s4 == 0)
+    assert np.abs(pos_err(y4, y)).max() < 1e-3 * np.median(e_net) + 1e-6
