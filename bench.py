@@ -289,6 +289,90 @@ def run_surrogate_study(args) -> dict:
     }
 
 
+def run_population(args) -> dict:
+    """Population evaluation: 32 NSGA-II generations x 4,096 candidates
+    (131,072 arcs) over seeded LEO departure states with dv U(-50, 50) m/s
+    per axis and coast U(0.5, 10) d, degree/order 50 + Vallado drag,
+    RK8(7)13M rtol 1e-12.  Each generation is one ragged device batch; the
+    field and the surrogate stay resident between generations.  Forms: plain
+    FP64, then the checked surrogate with a BF16 and with an FP8 network."""
+    import bqp_amd
+    from bqp_amd import inputs
+    from bqp_amd.population import DepartureProblem, Nsga2
+
+    pop, gens = args.pop, args.generations
+    w = inputs.workload("w1", n=pop, seed=7)
+    field = bqp_amd.kaula(50)
+    r_target = 6378137.0 + 800e3
+    forms = [f for f in args.forms.split(",") if f] if args.forms != "fp64,split,surrogate" else \
+        ["fp64", "surrogate_bf16", "surrogate_fp8"]
+    out, finals, sur = {}, {}, None
+    prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado")
+    for form in forms:
+        if form.startswith("surrogate"):
+            if sur is None:
+                prop.reconfigure(precision="fp64")
+                sur, feat, _, train_rep = train_surrogate(prop, args.sur_rows, args.sur_steps)
+            fmt = form.split("_")[1]
+            sur.pack(fmt, calib=feat.rows(0, 65536))
+            sur.set_check(args.sur_check, 1e-9)
+            prop.reconfigure(precision="surrogate")
+        else:
+            prop.reconfigure(precision=form)
+        prob = DepartureProblem(prop, w["y0"], w["bc"], r_target)
+        rng = np.random.Generator(np.random.PCG64(11))
+        t = time.perf_counter()
+        opt = Nsga2(prob, prob.lower, prob.upper, pop_size=pop, seed=11,
+                    initial=prob.initial_population(pop, rng))
+        first = prob.last[0].copy()
+
+        def progress(o):
+            print(f"[bench] population {form}: generation {o.generation + 1}/{gens}, device "
+                  f"{prob.device_s:.1f} s, {prob.orbit_days:.0f} orbit-days", file=sys.stderr, flush=True)
+
+        opt.run(gens - 1, callback=progress)
+        wall = time.perf_counter() - t
+        Xf, Ff = opt.front()
+        rec = {"orbit_days_s": prob.orbit_days / prob.device_s, "orbit_days": prob.orbit_days,
+               "device_s": prob.device_s, "wall_s": wall, "host_share": 1.0 - prob.device_s / wall,
+               "evaluations": opt.evaluations, "generations": opt.generation + 1, "rhs_evals": prob.evals,
+               "front_size": int(Xf.shape[0]), "front_min_dv_m_s": float(Ff[:, 0].min()),
+               "front_min_miss_km": float(Ff[:, 1].min())}
+        if form.startswith("surrogate"):
+            rec["surrogate_stats_last_generation"] = sur.last_stats()
+        finals[form] = first
+        out[form] = rec
+        print(f"[bench] population {form}: {rec}", file=sys.stderr, flush=True)
+    prop.close()
+    for form in forms:
+        if form != "fp64" and "fp64" in finals:
+            d = np.nan_to_num(np.linalg.norm(finals[form][:, :3] - finals["fp64"][:, :3], axis=1), nan=np.inf)
+            out[form].update({"gen0_pos_err_median_m": float(np.median(d)), "gen0_pos_err_max_m": float(d.max()),
+                              "speedup_vs_fp64": out["fp64"]["device_s"] / out[form]["device_s"]})
+    base = out.get("fp64") or next(iter(out.values()))
+    return {
+        "metric": "orbit-days/s, NSGA-II population evaluation: 32 generations x 4,096 arcs, deg-50 + drag",
+        "value": base["orbit_days_s"],
+        "unit": "orbit-days/s",
+        "n_gpus": 1,
+        "steps": gens,
+        "warmup": 0,
+        "ms_per_step": base["device_s"] / gens * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded LEO departures (PCG64 seed 7), dv U(-50,50) m/s per axis, coast U(0.5,10) d, "
+                "seeded Kaula degree-50 field",
+        "config": {"workload": f"{gens} NSGA-II generations x {pop} candidates, objectives |dv| and final-radius "
+                               f"miss vs {r_target / 1e3:.0f} km, RK8(7)13M rtol 1e-12, degree/order 50, Vallado "
+                               "drag; one ragged device batch per generation", "forms": forms},
+        "roofline": None,
+        "cpu_baseline": None,
+        "forms": out,
+    }
+
+
 def cpu_baseline_ragged(w, field, threads: int) -> dict:
     from oracle import oracle as orc
 
@@ -333,9 +417,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0)
     ap.add_argument("--launch", default="", help='JSON launch shape, e.g. {"waves_per_group": 2}')
-    ap.add_argument("--workload", default="w1", choices=["w1", "w2", "w3"],
+    ap.add_argument("--pop", type=int, default=4096, help="w4: population size")
+    ap.add_argument("--generations", type=int, default=32, help="w4: generations (incl. the initial one)")
+    ap.add_argument("--workload", default="w1", choices=["w1", "w2", "w3", "w4"],
                     help="w1: 65,536 LEO x 7 d deg-70 (the headline); w2: ragged 262,144 mixed orbits deg-20; "
-                         "w3: the surrogate study (training, MFMA inference rooflines, fallback, ablation)")
+                         "w3: the surrogate study (training, MFMA inference rooflines, fallback, ablation); "
+                         "w4: NSGA-II population evaluation (32 x 4,096 arcs, deg-50 + drag)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the propagation path runs on a single MI355X "
@@ -348,6 +435,9 @@ def main():
         return
     if args.workload == "w3":
         print(json.dumps(run_surrogate_study(args)))
+        return
+    if args.workload == "w4":
+        print(json.dumps(run_population(args)))
         return
 
     import bqp_amd
