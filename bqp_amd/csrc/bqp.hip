@@ -158,10 +158,53 @@ void pack_field(int N, int M, const double* C, const double* S, double mu, doubl
   }
 }
 
+// Rescaled copy of an (a, b) table (the GravEntryL form): in each column s_n
+// is the running product of the recursion coefficients a_n (exact formulas in
+// long double; s = 1 on the column's first entry), g_n = b_n / (a_n a_{n-1})
+// and the summation coefficients are multiplied by s_n.
+void pack_lambda(int N, int jmax, const std::vector<GravEntry>& ab, std::vector<GravEntryL>& out) {
+  out.clear();
+  out.reserve(ab.size());
+  auto a_nm = [](int n, int m) -> long double {
+    const long double dn = n, dm = m;
+    return sqrtl((2 * dn + 1) * (2 * dn - 1) / ((dn - dm) * (dn + dm)));
+  };
+  auto b_nm = [](int n, int m) -> long double {
+    if (n < m + 2) return 0.0L;
+    const long double dn = n, dm = m;
+    return sqrtl((2 * dn + 1) * (dn + dm - 1) * (dn - dm - 1) / ((dn - dm) * (dn + dm) * (2 * dn - 3)));
+  };
+  size_t idx = 0;
+  for (int j = 0; j <= jmax; ++j) {
+    const int nlo = j == 0 ? 1 : j;
+    long double s = 1.0L, aprev = 0.0L;
+    for (int n = nlo; n <= N + 1; ++n, ++idx) {
+      const GravEntry& e = ab[idx];
+      GravEntryL L{};
+      long double scale = 1.0L;
+      if (!(j >= 1 && n == j)) {  // recursion entry (the sectoral entry keeps s = 1)
+        const long double a = a_nm(n, j), b = b_nm(n, j);
+        s *= a;
+        scale = s;
+        L.g = aprev > 0.0L ? (double)(b / (a * aprev)) : 0.0;
+        aprev = a;
+      }
+      L.xv = (double)((long double)e.xv * scale);
+      L.xw = (double)((long double)e.xw * scale);
+      L.yv = (double)((long double)e.yv * scale);
+      L.yw = (double)((long double)e.yw * scale);
+      L.zv = (double)((long double)e.zv * scale);
+      L.zw = (double)((long double)e.zw * scale);
+      out.push_back(L);
+    }
+  }
+}
+
 FieldView field_view(const bqp_ctx* c) {
   FieldView f;
   f.tab = c->d_tab;
   f.tabf = c->d_tabf;
+  f.tabl = c->d_tabl;
   f.nmax = c->nmax;
   f.jmax = c->jmax;
   f.radius = c->radius;
@@ -220,11 +263,13 @@ void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, prop_fn ker
                      (int)n, c->d_first);
 }
 
-// Algorithmic FLOPs of one gravity evaluation (recursion + summation, FMA = 2):
-// column 0 costs 16 + 11 (N+1), column j >= 1 costs 20 + 20 (N+1-j).
+// Algorithmic FLOPs of one FP64 gravity evaluation in the rescaled recursion
+// (FMA = 2): set-up 16, column 0 9 per entry (g rho, U, three sums), column
+// j >= 1 20 for the sectoral step and entry plus 19 per recursion entry
+// (g rho, U and W, six sums).
 double grav_flops(int N, int jmax) {
-  double f = 16.0 + 11.0 * (N + 1.0);
-  for (int j = 1; j <= jmax; ++j) f += 20.0 + 20.0 * (N + 1.0 - j);
+  double f = 16.0 + 9.0 * (N + 1.0);
+  for (int j = 1; j <= jmax; ++j) f += 20.0 + 19.0 * (N + 1.0 - j);
   return f;
 }
 
@@ -279,7 +324,8 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
   if (sur) {
     const bqp_mlp_state* s = c->mlp;
     p.low = field_view(c);
-    p.low.tab = s->d_tab_low;
+    p.low.tab = nullptr;
+    p.low.tabl = s->d_tab_low;
     p.low.tabf = nullptr;
     p.low.nmax = std::min(s->nlow, c->nmax);
     p.low.jmax = s->low_jmax;
@@ -471,6 +517,7 @@ int32_t bqp_destroy(bqp_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)hipFree(c->d_tab);
   (void)hipFree(c->d_tabf);
+  (void)hipFree(c->d_tabl);
   (void)hipFree(c->d_queue);
   (void)hipFree(c->d_totals);
   (void)hipFree(c->d_first);
@@ -522,6 +569,8 @@ int32_t bqp_load_field(bqp_ctx* c, int32_t nmax, int32_t mmax, const double* cba
   std::vector<GravEntry> tab;
   int jmax = 0;
   pack_field(nmax, mmax, cbar, sbar, mu, radius, tab, jmax);
+  std::vector<GravEntryL> tabl;
+  pack_lambda(nmax, jmax, tab, tabl);
   std::vector<GravEntryF> tabf(tab.size());
   for (size_t i = 0; i < tab.size(); ++i) {
     const GravEntry& e = tab[i];
@@ -532,8 +581,10 @@ int32_t bqp_load_field(bqp_ctx* c, int32_t nmax, int32_t mmax, const double* cba
   bqp_mlp_release(c);  // a surrogate belongs to the field it was set up for
   (void)hipFree(c->d_tab);
   (void)hipFree(c->d_tabf);
+  (void)hipFree(c->d_tabl);
   c->d_tab = nullptr;
   c->d_tabf = nullptr;
+  c->d_tabl = nullptr;
   c->nmax = -1;
   c->h_cbar.assign(cbar, cbar + nt);
   c->h_sbar.assign(sbar, sbar + nt);
@@ -542,10 +593,14 @@ int32_t bqp_load_field(bqp_ctx* c, int32_t nmax, int32_t mmax, const double* cba
   tab.push_back(GravEntry{});
   tabf.push_back(GravEntryF{});
   tabf.push_back(GravEntryF{});
+  tabl.push_back(GravEntryL{});
+  tabl.push_back(GravEntryL{});
   HIP_TRY(c, hipMalloc(&c->d_tab, tab.size() * sizeof(GravEntry)));
   HIP_TRY(c, hipMalloc(&c->d_tabf, tabf.size() * sizeof(GravEntryF)));
   HIP_TRY(c, hipMemcpy(c->d_tab, tab.data(), tab.size() * sizeof(GravEntry), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMemcpy(c->d_tabf, tabf.data(), tabf.size() * sizeof(GravEntryF), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMalloc(&c->d_tabl, tabl.size() * sizeof(GravEntryL)));
+  HIP_TRY(c, hipMemcpy(c->d_tabl, tabl.data(), tabl.size() * sizeof(GravEntryL), hipMemcpyHostToDevice));
   c->n_entries = tab.size();
   c->nmax = nmax;
   c->mmax = mmax;
@@ -732,7 +787,8 @@ int32_t bqp_surrogate_dataset_dev(bqp_ctx* c, int64_t n, uint64_t seed, double* 
   if (n == 0) return BQP_OK;
   FieldView full = field_view(c);
   FieldView low = full;
-  low.tab = s->d_tab_low;
+  low.tab = nullptr;
+  low.tabl = s->d_tab_low;
   low.tabf = nullptr;
   low.nmax = std::min(s->nlow, c->nmax);
   low.jmax = s->low_jmax;
@@ -763,15 +819,17 @@ int32_t bqp_surrogate_init(bqp_ctx* c, int32_t nlow, double r_lo, double r_hi, u
   s->check_every = 100;
   s->threshold = 1e-9;
   std::vector<GravEntry> tab;
+  std::vector<GravEntryL> tabl;
   int jmax = 0;
   pack_field(nlow, std::min(c->mmax, nlow), c->h_cbar.data(), c->h_sbar.data(), c->mu, c->radius, tab, jmax);
-  tab.push_back(GravEntry{});
-  tab.push_back(GravEntry{});
+  pack_lambda(nlow, jmax, tab, tabl);
+  tabl.push_back(GravEntryL{});
+  tabl.push_back(GravEntryL{});
   (void)hipStreamSynchronize(c->stream);
   (void)hipFree(s->d_tab_low);
   s->d_tab_low = nullptr;
-  HIP_TRY(c, hipMalloc(&s->d_tab_low, tab.size() * sizeof(GravEntry)));
-  HIP_TRY(c, hipMemcpy(s->d_tab_low, tab.data(), tab.size() * sizeof(GravEntry), hipMemcpyHostToDevice));
+  HIP_TRY(c, hipMalloc(&s->d_tab_low, tabl.size() * sizeof(GravEntryL)));
+  HIP_TRY(c, hipMemcpy(s->d_tab_low, tabl.data(), tabl.size() * sizeof(GravEntryL), hipMemcpyHostToDevice));
   s->low_jmax = jmax;
   int rc = bqp_mlp_alloc_init(c, seed);
   if (rc) return rc;

@@ -38,6 +38,16 @@ struct GravEntryF {
   float a, b, xv, xw, yv, yw, zv, zw;
 };
 
+// Rescaled form of the FP64 table.  Within column m, with s_n the product of
+// the recursion coefficients a_km (k = m+1..n; s = 1 on the column's first
+// entry), U(n) = V(n)/s_n obeys
+//   U(n) = zt U(n-1) - g_n rho U(n-2),   g_n = b_n / (a_n a_{n-1}),
+// and the summation coefficients absorb s_n (xv = XV_n s_n, ...).  One
+// multiply per entry fewer than the (a, b) form, and 7 doubles instead of 8.
+struct GravEntryL {
+  double g, xv, xw, yv, yw, zv, zw;
+};
+
 // The table is read through the constant address space: loads from it are
 // invariant by definition, so uniform reads compile to scalar (SMEM) loads
 // even in kernels that also store to global memory.
@@ -45,9 +55,11 @@ struct GravEntryF {
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const __attribute__((address_space(4))) GravEntry* CEntry;
 typedef const __attribute__((address_space(4))) GravEntryF* CEntryF;
+typedef const __attribute__((address_space(4))) GravEntryL* CEntryL;
 #else
 typedef const GravEntry* CEntry;
 typedef const GravEntryF* CEntryF;
+typedef const GravEntryL* CEntryL;
 #endif
 
 // Offset of column j in the entry table of a degree-N field.  Column 0 holds
@@ -61,8 +73,9 @@ __host__ __device__ inline int col_offset(int N, int j) {
 constexpr int SPLIT_N = 5;
 
 struct FieldView {
-  const GravEntry* tab;    // FP64 entries, all columns
+  const GravEntry* tab;    // FP64 entries, all columns ((a, b) form: split precision head)
   const GravEntryF* tabf;  // FP32 copy of the same entries (split precision)
+  const GravEntryL* tabl;  // FP64 entries in the rescaled form (plain FP64 recursion)
   int nmax;                // field degree N
   int jmax;                // last column = min(N+1, M+1)
   double radius;
@@ -85,20 +98,29 @@ constexpr int SECT_MAX = 208;
 static __constant__ double c_sect[SECT_MAX];
 static __constant__ float c_sectf[SECT_MAX];
 
-// One recursion step of a column: V(n) from V(n-1), V(n-2), then the
-// contributions of entry e.  Writes the new value over V(n-2)'s register so a
-// pair of steps needs no moves.
-#define BQP_COL_STEP(E, VOUT, VPREV, WOUT, WPREV)   \
-  {                                                 \
-    const double A_ = (E).a * zt, B_ = (E).b * rho; \
-    VOUT = fma(A_, VPREV, -(B_ * VOUT));            \
-    WOUT = fma(A_, WPREV, -(B_ * WOUT));            \
-    sx = fma((E).xv, VOUT, sx);                     \
-    sx = fma((E).xw, WOUT, sx);                     \
-    sy = fma((E).yv, VOUT, sy);                     \
-    sy = fma((E).yw, WOUT, sy);                     \
-    sz = fma((E).zv, VOUT, sz);                     \
-    sz = fma((E).zw, WOUT, sz);                     \
+// One recursion step of a column in the rescaled form: U(n) from U(n-1),
+// U(n-2), then the contributions of entry e (11 FP64 operations).  Writes
+// the new value over U(n-2)'s register so a pair of steps needs no moves.
+#define BQP_COL_STEP(E, VOUT, VPREV, WOUT, WPREV) \
+  {                                               \
+    const double C_ = (E).g * rho;                \
+    VOUT = fma(zt, VPREV, -(C_ * VOUT));          \
+    WOUT = fma(zt, WPREV, -(C_ * WOUT));          \
+    sx = fma((E).xv, VOUT, sx);                   \
+    sx = fma((E).xw, WOUT, sx);                   \
+    sy = fma((E).yv, VOUT, sy);                   \
+    sy = fma((E).yw, WOUT, sy);                   \
+    sz = fma((E).zv, VOUT, sz);                   \
+    sz = fma((E).zw, WOUT, sz);                   \
+  }
+// Column 0 (W = 0): the V half only.
+#define BQP_COL0_STEP(E, VOUT, VPREV)    \
+  {                                      \
+    const double C_ = (E).g * rho;       \
+    VOUT = fma(zt, VPREV, -(C_ * VOUT)); \
+    sx = fma((E).xv, VOUT, sx);          \
+    sy = fma((E).yv, VOUT, sy);          \
+    sz = fma((E).zv, VOUT, sz);          \
   }
 
 // FP64 gravity, body-fixed position (m) -> acceleration (m/s^2), for the
@@ -114,27 +136,26 @@ __device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, d
   double vs = R / sqrt(r2), ws = 0.0;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   const int top = f.nmax + 1;
-  CEntry e = (CEntry)f.tab;
+  CEntryL e = (CEntryL)f.tabl;
   if (col_owner<G>(0) == g) {
-    // column 0: W = 0 throughout
-    double v1 = vs, v2 = 0.0, w1 = 0.0, w2 = 0.0;
-    CEntry r = e;
+    double v1 = vs, v2 = 0.0;
+    CEntryL r = e;
     int k = top;
     for (; k >= 4; k -= 4, r += 4) {
-      const GravEntry e0 = r[0], e1 = r[1], e2 = r[2], e3 = r[3];
-      BQP_COL_STEP(e0, v2, v1, w2, w1);
-      BQP_COL_STEP(e1, v1, v2, w1, w2);
-      BQP_COL_STEP(e2, v2, v1, w2, w1);
-      BQP_COL_STEP(e3, v1, v2, w1, w2);
+      const GravEntryL e0 = r[0], e1 = r[1], e2 = r[2], e3 = r[3];
+      BQP_COL0_STEP(e0, v2, v1);
+      BQP_COL0_STEP(e1, v1, v2);
+      BQP_COL0_STEP(e2, v2, v1);
+      BQP_COL0_STEP(e3, v1, v2);
     }
     for (; k >= 2; k -= 2, r += 2) {
-      const GravEntry e0 = r[0], e1 = r[1];
-      BQP_COL_STEP(e0, v2, v1, w2, w1);
-      BQP_COL_STEP(e1, v1, v2, w1, w2);
+      const GravEntryL e0 = r[0], e1 = r[1];
+      BQP_COL0_STEP(e0, v2, v1);
+      BQP_COL0_STEP(e1, v1, v2);
     }
     if (k) {
-      const GravEntry e0 = r[0];
-      BQP_COL_STEP(e0, v2, v1, w2, w1);
+      const GravEntryL e0 = r[0];
+      BQP_COL0_STEP(e0, v2, v1);
     }
   }
   e += top;
@@ -147,11 +168,11 @@ __device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, d
     const int len = top + 1 - j;
     if (col_owner<G>(j) == g) {
       double v1 = vs, w1 = ws, v2 = 0.0, w2 = 0.0;
-      CEntry r = e + 1;
+      CEntryL r = e + 1;
       int k = len - 1;
       {
         // sectoral entry, loaded with the first batch of recursion entries
-        const GravEntry es = e[0];
+        const GravEntryL es = e[0];
         sx = fma(es.xv, vs, sx);
         sx = fma(es.xw, ws, sx);
         sy = fma(es.yv, vs, sy);
@@ -160,19 +181,19 @@ __device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, d
         sz = fma(es.zw, ws, sz);
       }
       for (; k >= 4; k -= 4, r += 4) {
-        const GravEntry e0 = r[0], e1 = r[1], e2 = r[2], e3 = r[3];
+        const GravEntryL e0 = r[0], e1 = r[1], e2 = r[2], e3 = r[3];
         BQP_COL_STEP(e0, v2, v1, w2, w1);
         BQP_COL_STEP(e1, v1, v2, w1, w2);
         BQP_COL_STEP(e2, v2, v1, w2, w1);
         BQP_COL_STEP(e3, v1, v2, w1, w2);
       }
       for (; k >= 2; k -= 2, r += 2) {
-        const GravEntry e0 = r[0], e1 = r[1];
+        const GravEntryL e0 = r[0], e1 = r[1];
         BQP_COL_STEP(e0, v2, v1, w2, w1);
         BQP_COL_STEP(e1, v1, v2, w1, w2);
       }
       if (k) {
-        const GravEntry e0 = r[0];
+        const GravEntryL e0 = r[0];
         BQP_COL_STEP(e0, v2, v1, w2, w1);
       }
     }

@@ -26,7 +26,7 @@ struct bqp_mlp_state {
   float* d_m = nullptr;
   float* d_v = nullptr;
   int64_t adam_t = 0;
-  bqp::GravEntry* d_tab_low = nullptr;  // degree <= nlow, same packing as the full table
+  bqp::GravEntryL* d_tab_low = nullptr;  // degree <= nlow, rescaled form like the full table
   int low_jmax = 0;
   int packed_fmt = -1;  // -1: nothing packed
   void* d_packed = nullptr;
@@ -52,6 +52,7 @@ struct bqp_ctx {
   int nmax = -1, mmax = -1, jmax = -1;
   double mu = 0.0, radius = 0.0;
   bqp::GravEntry* d_tab = nullptr;
+  bqp::GravEntryL* d_tabl = nullptr;
   bqp::GravEntryF* d_tabf = nullptr;
   size_t n_entries = 0;
   std::vector<double> h_cbar, h_sbar;  // host copy of the loaded coefficients
