@@ -417,6 +417,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=0)
     ap.add_argument("--launch", default="", help='JSON launch shape, e.g. {"waves_per_group": 2}')
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="after the run, one device-to-device copy of this many MiB (a known-size kernel for "
+                         "calibrating HBM byte counters)")
     ap.add_argument("--pop", type=int, default=4096, help="w4: population size")
     ap.add_argument("--generations", type=int, default=32, help="w4: generations (incl. the initial one)")
     ap.add_argument("--workload", default="w1", choices=["w1", "w2", "w3", "w4"],
@@ -541,7 +544,7 @@ def main():
             threads = min(16, len(os.sched_getaffinity(0)))
         except AttributeError:
             threads = min(16, os.cpu_count() or 1)
-        rows = args.cpu_rows or 4 * threads
+        rows = args.cpu_rows or 8 * threads
         cpu = cpu_baseline(w, field, rows, 1.0, threads)
         print(f"[bench] cpu baseline {cpu['value']:.3f} orbit-days/s on {threads} threads", file=sys.stderr)
 
@@ -576,7 +579,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": base["achieved_tflops"] / FP64_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic() if (n == 65536 and args.days == 7.0) else None,
             "kernel": "bqp::propagate_kernel<RK87, FP64, G>",
             "kernel_ms": base["kernel_ms"],
             "flops_per_rhs": base["flops_per_rhs"],
@@ -586,6 +589,36 @@ def main():
         "forms": forms_out,
     }
     print(json.dumps(line))
+    if args.copy_mib:
+        copy_calibration(args.copy_mib)
+
+
+def copy_calibration(mib: int):
+    """One device-to-device copy of `mib` MiB: reads and writes exactly that
+    many bytes, the reference for scaling HBM byte counters."""
+    import ctypes as C
+
+    from bqp_amd.device import D2D, DeviceArray, hip, synchronize
+
+    n = (mib << 20) // 8
+    a = DeviceArray((n,))
+    b = DeviceArray((n,))
+    synchronize()
+    rc = hip().hipMemcpy(C.c_void_p(b.ptr), C.c_void_p(a.ptr), a.nbytes, D2D)
+    synchronize()
+    print(f"[bench] calibration copy of {mib} MiB ({a.nbytes} bytes), rc {rc}", file=sys.stderr, flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per launch of the propagation kernel on this workload, from
+    the committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled
+    by the copy-kernel calibration); None if absent."""
+    path = os.path.join(ROOT, "profiles", "propagate_rk87_deg70_leo65536_7d_traffic.json")
+    try:
+        with open(path) as fh:
+            return float(json.load(fh)["hbm_bytes_per_launch"])
+    except (OSError, ValueError, KeyError, TypeError):
+        return None
 
 
 if __name__ == "__main__":

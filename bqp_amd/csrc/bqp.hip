@@ -558,7 +558,7 @@ int32_t bqp_load_field(bqp_ctx* c, int32_t nmax, int32_t mmax, const double* cba
                        double mu, double radius) {
   if (!c) return BQP_E_INVALID;
   if (!cbar || !sbar) return fail(c, BQP_E_INVALID, "cbar/sbar must not be NULL");
-  if (nmax < 0 || nmax > SECT_MAX - 3 || mmax < 0 || mmax > nmax)
+  if (nmax < 0 || nmax > 200 || mmax < 0 || mmax > nmax)
     return fail(c, BQP_E_INVALID, "need 0 <= mmax <= nmax <= 200");
   if (!(mu > 0.0) || !(radius > 0.0) || !std::isfinite(mu) || !std::isfinite(radius))
     return fail(c, BQP_E_INVALID, "mu and radius must be finite and > 0");

@@ -96,6 +96,16 @@ __device__ __forceinline__ int col_owner(int j) {
 // cache, not the first entry of every column.
 constexpr int SECT_MAX = 208;
 static __constant__ double c_sect[SECT_MAX];
+
+// four consecutive sectoral factors, read with one scalar load
+struct Sect4 {
+  double v[4];
+};
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(4))) Sect4* CSect4;
+#else
+typedef const Sect4* CSect4;
+#endif
 static __constant__ float c_sectf[SECT_MAX];
 
 // One recursion step of a column in the rescaled form: U(n) from U(n-1),
@@ -107,11 +117,11 @@ static __constant__ float c_sectf[SECT_MAX];
     VOUT = fma(zt, VPREV, -(C_ * VOUT));          \
     WOUT = fma(zt, WPREV, -(C_ * WOUT));          \
     sx = fma((E).xv, VOUT, sx);                   \
-    sx = fma((E).xw, WOUT, sx);                   \
+    tx = fma((E).xw, WOUT, tx);                   \
     sy = fma((E).yv, VOUT, sy);                   \
-    sy = fma((E).yw, WOUT, sy);                   \
+    ty = fma((E).yw, WOUT, ty);                   \
     sz = fma((E).zv, VOUT, sz);                   \
-    sz = fma((E).zw, WOUT, sz);                   \
+    tz = fma((E).zw, WOUT, tz);                   \
   }
 // Column 0 (W = 0): the V half only.
 #define BQP_COL0_STEP(E, VOUT, VPREV)    \
@@ -134,7 +144,9 @@ __device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, d
   const double rr = 1.0 / r2;
   const double xt = x * R * rr, yt = y * R * rr, zt = z * R * rr, rho = R * R * rr;
   double vs = R / sqrt(r2), ws = 0.0;
-  double sx = 0.0, sy = 0.0, sz = 0.0;
+  // V-term and W-term sums are kept apart (two accumulation chains per
+  // component, half the dependent-FMA latency per entry)
+  double sx = 0.0, sy = 0.0, sz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
   const int top = f.nmax + 1;
   CEntryL e = (CEntryL)f.tabl;
   if (col_owner<G>(0) == g) {
@@ -159,49 +171,58 @@ __device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, d
     }
   }
   e += top;
-  for (int j = 1; j <= f.jmax; ++j) {
-    const double s = c_sect[j];
-    const double nv = s * fma(xt, vs, -(yt * ws));
-    const double nw = s * fma(xt, ws, yt * vs);
-    vs = nv;
-    ws = nw;
-    const int len = top + 1 - j;
-    if (col_owner<G>(j) == g) {
-      double v1 = vs, w1 = ws, v2 = 0.0, w2 = 0.0;
-      CEntryL r = e + 1;
-      int k = len - 1;
-      {
-        // sectoral entry, loaded with the first batch of recursion entries
-        const GravEntryL es = e[0];
-        sx = fma(es.xv, vs, sx);
-        sx = fma(es.xw, ws, sx);
-        sy = fma(es.yv, vs, sy);
-        sy = fma(es.yw, ws, sy);
-        sz = fma(es.zv, vs, sz);
-        sz = fma(es.zw, ws, sz);
+  // The sectoral factors are read four per scalar load (one wait per four
+  // columns instead of one per column): every wave walks the whole sectoral
+  // chain, whether it owns the columns or not.
+  for (int j0 = 1; j0 <= f.jmax; j0 += 4) {
+    const Sect4 sq = *(CSect4)(c_sect + j0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int j = j0 + k;
+      if (j > f.jmax) break;
+      const double s = sq.v[k];
+      const double nv = s * fma(xt, vs, -(yt * ws));
+      const double nw = s * fma(xt, ws, yt * vs);
+      vs = nv;
+      ws = nw;
+      const int len = top + 1 - j;
+      if (col_owner<G>(j) == g) {
+        double v1 = vs, w1 = ws, v2 = 0.0, w2 = 0.0;
+        CEntryL r = e + 1;
+        int kk = len - 1;
+        {
+          // sectoral entry, loaded with the first batch of recursion entries
+          const GravEntryL es = e[0];
+          sx = fma(es.xv, vs, sx);
+          tx = fma(es.xw, ws, tx);
+          sy = fma(es.yv, vs, sy);
+          ty = fma(es.yw, ws, ty);
+          sz = fma(es.zv, vs, sz);
+          tz = fma(es.zw, ws, tz);
+        }
+        for (; kk >= 4; kk -= 4, r += 4) {
+          const GravEntryL e0 = r[0], e1 = r[1], e2 = r[2], e3 = r[3];
+          BQP_COL_STEP(e0, v2, v1, w2, w1);
+          BQP_COL_STEP(e1, v1, v2, w1, w2);
+          BQP_COL_STEP(e2, v2, v1, w2, w1);
+          BQP_COL_STEP(e3, v1, v2, w1, w2);
+        }
+        for (; kk >= 2; kk -= 2, r += 2) {
+          const GravEntryL e0 = r[0], e1 = r[1];
+          BQP_COL_STEP(e0, v2, v1, w2, w1);
+          BQP_COL_STEP(e1, v1, v2, w1, w2);
+        }
+        if (kk) {
+          const GravEntryL e0 = r[0];
+          BQP_COL_STEP(e0, v2, v1, w2, w1);
+        }
       }
-      for (; k >= 4; k -= 4, r += 4) {
-        const GravEntryL e0 = r[0], e1 = r[1], e2 = r[2], e3 = r[3];
-        BQP_COL_STEP(e0, v2, v1, w2, w1);
-        BQP_COL_STEP(e1, v1, v2, w1, w2);
-        BQP_COL_STEP(e2, v2, v1, w2, w1);
-        BQP_COL_STEP(e3, v1, v2, w1, w2);
-      }
-      for (; k >= 2; k -= 2, r += 2) {
-        const GravEntryL e0 = r[0], e1 = r[1];
-        BQP_COL_STEP(e0, v2, v1, w2, w1);
-        BQP_COL_STEP(e1, v1, v2, w1, w2);
-      }
-      if (k) {
-        const GravEntryL e0 = r[0];
-        BQP_COL_STEP(e0, v2, v1, w2, w1);
-      }
+      e += len;
     }
-    e += len;
   }
-  ax = sx;
-  ay = sy;
-  az = sz;
+  ax = sx + tx;
+  ay = sy + ty;
+  az = sz + tz;
 }
 
 // Split-precision gravity: entries with n <= SPLIT_N (degree <= 4) in FP64,

@@ -466,23 +466,16 @@ propagate_kernel(const PropParams p) {
         stepping = false;
       }
     }
-    // stage 1 (k1): kept across rejects, reused after FSAL accepts
-    {
-      const bool needk = stepping && !have_k1;
-      if (wave_ballot(needk) != 0ull) {
-        double a0[3];
-        if (PREC >= 2) sl.check_now = p.sur_check_every > 0 && (evals % p.sur_check_every) == 0;
-        accel_eval<PREC, G, S>(p, g, lane, lds, sA, 0, needk, bc, t, y, a0, sX, sO, sFlag, sl);
-        if (needk) {
-          ++evals;
-          have_k1 = true;
-        }
-      }
-    }
     if (wave_ballot(stepping) == 0ull) continue;
+    // stage 1 (k1) is kept across rejects and reused after FSAL accepts, so
+    // only lanes without it evaluate stage 0; the stage loop holds the one
+    // inlined copy of the RHS (stage 0: c = 0 and no a terms, so ys = y, t)
+    const bool needk = stepping && !have_k1;
     const double hh = hs * hs;
 #pragma clang loop unroll(disable)
-    for (int s = 1; s < S; ++s) {
+    for (int s = 0; s < S; ++s) {
+      const bool st = (s == 0) ? needk : stepping;
+      if (s == 0 && wave_ballot(needk) == 0ull) continue;
       double ys[6];
       const double hc = hs * SO.csum[s];
 #pragma unroll
@@ -509,9 +502,10 @@ propagate_kernel(const PropParams p) {
       }
       double as[3];
       if (PREC >= 2) sl.check_now = p.sur_check_every > 0 && (evals % p.sur_check_every) == 0;
-      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, stepping, bc, t + T.c[s] * hs, ys, as, sX, sO, sFlag, sl);
-      if (stepping) ++evals;
+      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, st, bc, t + T.c[s] * hs, ys, as, sX, sO, sFlag, sl);
+      if (st) ++evals;
     }
+    if (needk) have_k1 = true;
     if (!stepping) continue;
 
     // solution and error estimates (Kahan-compensated error sums)
