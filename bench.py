@@ -544,7 +544,7 @@ def main():
             threads = min(16, len(os.sched_getaffinity(0)))
         except AttributeError:
             threads = min(16, os.cpu_count() or 1)
-        rows = args.cpu_rows or 8 * threads
+        rows = args.cpu_rows or 24 * threads
         cpu = cpu_baseline(w, field, rows, 1.0, threads)
         print(f"[bench] cpu baseline {cpu['value']:.3f} orbit-days/s on {threads} threads", file=sys.stderr)
 

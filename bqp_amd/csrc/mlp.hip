@@ -168,18 +168,32 @@ __global__ void __launch_bounds__(256) loss_final_kernel(const double* __restric
   }
 }
 
-// column sums of dZ (B x N) -> db (N); block = 64 columns x 4 row groups
+// Column sums of dZ (B x N) -> db (N) in two fixed-order passes: block
+// (cb, rs) sums rows rs, rs + RS, ... of 64 columns into part[rs][.], then
+// colsum_final_kernel adds the RS partials.
+constexpr int COLSUM_SPLITS = 64;
 __global__ void __launch_bounds__(256) colsum_kernel(const float* __restrict__ dZ, int B, int N,
-                                                     float* __restrict__ db) {
+                                                     float* __restrict__ part) {
   __shared__ float red[4][64];
   const int c = blockIdx.x * 64 + (threadIdx.x & 63);
   const int rg = threadIdx.x >> 6;
+  const int rs = blockIdx.y;
   float s = 0.f;
   if (c < N)
-    for (int r = rg; r < B; r += 4) s += dZ[(long long)r * N + c];
+    for (int r = rs * 4 + rg; r < B; r += 4 * COLSUM_SPLITS) s += dZ[(long long)r * N + c];
   red[rg][threadIdx.x & 63] = s;
   __syncthreads();
-  if (rg == 0 && c < N) db[c] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+  if (rg == 0 && c < N)
+    part[(long long)rs * N + c] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+__global__ void __launch_bounds__(256) colsum_final_kernel(const float* __restrict__ part, int N,
+                                                           float* __restrict__ db) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= N) return;
+  float s = 0.f;
+  for (int rs = 0; rs < COLSUM_SPLITS; ++rs) s += part[(long long)rs * N + c];
+  db[c] = s;
 }
 
 // dW = sum over splits of the partial products (fixed order)
@@ -270,7 +284,7 @@ int fail(bqp_ctx* c, int code, const std::string& m) { return bqp_fail(c, code, 
 
 // training scratch for minibatches of up to `cap` rows
 struct Work {
-  float *X, *T, *Y, *dY, *dZa, *dZb, *part;
+  float *X, *T, *Y, *dY, *dZa, *dZb, *part, *cpart;
   float* Z[4];
   float* H[4];
   double* lpart;
@@ -302,6 +316,7 @@ Work carve(char* base, int cap, size_t* total) {
     w.H[l] = take((size_t)cap * MLP_WIDTH);
   }
   w.part = take((size_t)SPLITS * MLP_WIDTH * MLP_WIDTH);
+  w.cpart = take((size_t)COLSUM_SPLITS * MLP_WIDTH);
   w.lsteps = take(MAX_LOSS_STEPS);
   w.amax = take(1024);
   w.lpart = (double*)take(2 * ((size_t)cap / 256 + 2));
@@ -365,7 +380,10 @@ void loss_backward(hipStream_t st, const float* P, float* G, const float* X, con
     const long long cnt = (long long)outd * in;
     hipLaunchKernelGGL(split_reduce_kernel, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st, w.part, z, cnt,
                        G + mlp_w_off(l));
-    hipLaunchKernelGGL(colsum_kernel, dim3((outd + 63) / 64), dim3(256), 0, st, dZ, B, outd, G + mlp_b_off(l));
+    hipLaunchKernelGGL(colsum_kernel, dim3((outd + 63) / 64, COLSUM_SPLITS), dim3(256), 0, st, dZ, B, outd,
+                       w.cpart);
+    hipLaunchKernelGGL(colsum_final_kernel, dim3((outd + 255) / 256), dim3(256), 0, st, w.cpart, outd,
+                       G + mlp_b_off(l));
     if (l > 0) {
       float* dzn = (dZ == w.dZa) ? w.dZb : w.dZa;
       // dZ_{l-1} = (dZ_l (B x out) . W_l (out x in)) * SiLU'(Z_{l-1})
