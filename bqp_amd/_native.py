@@ -27,6 +27,7 @@ METHODS = {"dopri5": 0, "dop853": 1, "rk87": 2, "dopri87": 2, "rk8(7)13m": 2}
 CONTROLS = {"scaled": 0, "absolute": 1, "compat": 2}
 DRAGS = {"none": 0, "vallado": 1, "compat": 2}
 PRECISIONS = {"fp64": 0, "split": 1, "surrogate": 2}
+FORMULATIONS = {"cowell": 0, "encke": 1}
 MLP_FORMATS = {"fp32": 0, "bf16": 1, "fp8": 2}
 
 
@@ -45,7 +46,7 @@ class Config(C.Structure):
         ("h_max", C.c_double),
         ("max_steps", C.c_int64),
         ("max_rejects", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("formulation", C.c_int32),
         ("omega", C.c_double),
         ("theta0", C.c_double),
         ("r_atm", C.c_double),

@@ -11,7 +11,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRCS = [os.path.join(HERE, "csrc", f) for f in
-        ("prop_sur.hip", "prop_sur_fp8.hip", "prop_fp64.hip", "prop_split.hip", "bqp.hip", "mlp.hip")]
+        ("prop_sur.hip", "prop_sur_fp8.hip", "prop_encke.hip", "prop_fp64.hip", "prop_split.hip", "bqp.hip",
+         "mlp.hip")]
 OUT = os.path.join(HERE, "libbqp.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -74,6 +74,11 @@ bool valid_config(const bqp_config& c, std::string& why) {
   if (c.control < 0 || c.control > 2) { why = "control must be 0, 1 or 2"; return false; }
   if (c.precision < 0 || c.precision > 2) { why = "precision must be 0 (fp64), 1 (split) or 2 (surrogate)"; return false; }
   if (c.drag < 0 || c.drag > 2) { why = "drag must be 0, 1 or 2"; return false; }
+  if (c.formulation < 0 || c.formulation > 1) { why = "formulation must be 0 (Cowell) or 1 (Encke)"; return false; }
+  if (c.formulation == BQP_FORMULATION_ENCKE && c.precision != BQP_PRECISION_FP64) {
+    why = "the Encke formulation runs in FP64 (precision 0)";
+    return false;
+  }
   if (c.control == BQP_CONTROL_SCALED &&
       !(c.rtol >= 0.0 && c.atol_pos >= 0.0 && c.atol_vel >= 0.0 && (c.rtol > 0 || (c.atol_pos > 0 && c.atol_vel > 0)))) {
     why = "scaled control needs rtol >= 0, atol >= 0, not all zero";
@@ -216,6 +221,7 @@ FieldView field_view(const bqp_ctx* c) {
 prop_fn pick_kernel(int method, int prec, int G) {
   if (prec == 0) return pick_fp64(method, G);
   if (prec == 1) return pick_split(method, G);
+  if (prec == 4) return pick_encke(method, G);
   return pick_sur(method, prec, G);
 }
 
@@ -288,6 +294,7 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
       return fail(c, BQP_E_INVALID, "surrogate precision needs packed weights (bqp_surrogate_pack)");
     kprec = s->packed_fmt == BQP_MLP_FP8 ? 3 : 2;
   }
+  if (cfg.formulation == BQP_FORMULATION_ENCKE) kprec = 4;
   const int G = c->launch.waves_per_group > 0 ? c->launch.waves_per_group : default_g(c, n);
   prop_fn kern = pick_kernel(cfg.method, kprec, G);
   if (!kern) return fail(c, BQP_E_INVALID, "waves_per_group must be 1, 2, 4 or 8");
@@ -306,6 +313,7 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
   p.queue = c->d_queue;
   p.n = (int)n;
   p.field = field_view(c);
+  p.mu = c->mu;
   p.model.omega = cfg.omega;
   p.model.theta0 = cfg.theta0;
   p.model.r_atm = cfg.r_atm;
@@ -503,7 +511,8 @@ int32_t bqp_create(const bqp_config* cfg, bqp_ctx** out) {
         hipMemcpyToSymbol(HIP_SYMBOL(c_sect), sect, sizeof(sect)) != hipSuccess ||
         hipMemcpyToSymbol(HIP_SYMBOL(c_sectf), sectf, sizeof(sectf)) != hipSuccess ||
         upload_fp64(so, sect, sectf) != hipSuccess || upload_split(so, sect, sectf) != hipSuccess ||
-        upload_sur(so, sect, sectf) != hipSuccess || upload_sur_fp8(so, sect, sectf) != hipSuccess) {
+        upload_sur(so, sect, sectf) != hipSuccess || upload_sur_fp8(so, sect, sectf) != hipSuccess ||
+        upload_encke(so, sect, sectf) != hipSuccess) {
       bqp_destroy(c);
       return BQP_E_DEVICE;
     }

@@ -130,11 +130,12 @@ struct PropParams {
   double t0c;
   FieldView field;
   ModelView model;
+  double mu;                      // GM of the field (m^3/s^2): Encke baseline
   int control;
   int max_steps;
   int max_rejects;
   double rtol, atol_pos, atol_vel, eps, h0, h_min, h_max;
-  // learned surrogate (PREC >= 2 only)
+  // learned surrogate (PREC 2 and 3 only)
   FieldView low;                  // exact part, degree <= nlow
   MlpPacked mlp;                  // packed network
   double sur_r_lo, sur_r_hi;      // radius band (m)
@@ -143,7 +144,7 @@ struct PropParams {
   unsigned long long* sur_totals; // net, exact, checks, failures, band
 };
 
-// Per-lane surrogate bookkeeping (PREC >= 2).
+// Per-lane surrogate bookkeeping (PREC 2 and 3).
 struct SurLane {
   bool exact = false;       // row switched to the exact field after a failed check
   bool check_now = false;   // this evaluation is a sampled check
@@ -232,6 +233,90 @@ __device__ __forceinline__ void sur_combine(const PropParams& p, SurLane& sl, bo
   }
 }
 
+// ------------------------------------------------------------- Encke
+// PREC 4: Encke formulation in FP64 (the reference's native one,
+// LY/src/rhs.rs:3885-4118, segment driver LY/src/integrator.rs:3418-3563).
+// The integrated state is the deviation d from a two-body baseline started
+// from the total state at the segment start; segments are at most
+// ENCKE_SEGMENT_S long (MAX_RECTIFICATION_SEGMENT_S, integrator.rs:236) and
+// each starts a fresh controller.
+constexpr double ENCKE_SEGMENT_S = 5400.0;
+
+struct EnckeLane {
+  double base[6];  // total state at the segment start (inertial, m, m/s)
+  double t0;       // segment start time
+  double x;        // eccentric-anomaly change of the last baseline solve (warm start)
+  double seg_end;  // segment end time
+};
+
+// Two-body state dt after s0 by Lagrange coefficients in the
+// eccentric-anomaly change x (Battin 1999 §4.5):
+//   n dt = x + s1 (1 - cos x) - c1 sin x,  s1 = r0.v0/sqrt(mu a),  c1 = 1 - r0/a,
+// Halley iterations from the last solution (or n dt), |dx| <= 1e-15 max(1,|x|).
+// Returns false if s0 is not an ellipse or the solve does not converge.
+__device__ __forceinline__ bool two_body(double mu, const double* s0, double dt, double* out, double& xio) {
+  const double r0 = sqrt(fma(s0[0], s0[0], fma(s0[1], s0[1], s0[2] * s0[2])));
+  const double v2 = fma(s0[3], s0[3], fma(s0[4], s0[4], s0[5] * s0[5]));
+  const double rv = fma(s0[0], s0[3], fma(s0[1], s0[4], s0[2] * s0[5]));
+  const double ainv = 2.0 / r0 - v2 / mu;
+  if (!(ainv > 0.0) || !isfinite(ainv)) {
+    for (int i = 0; i < 6; ++i) out[i] = __builtin_nan("");
+    return false;
+  }
+  const double a = 1.0 / ainv;
+  const double sqa = sqrt(a);
+  const double n = sqrt(mu * ainv * ainv * ainv);
+  const double s1 = rv / (sqrt(mu) * sqa);
+  const double c1 = 1.0 - r0 * ainv;
+  const double M = n * dt;
+  double x = isfinite(xio) ? xio : M;
+  bool ok = false;
+  for (int it = 0; it < 50; ++it) {
+    double sx, cx;
+    sincos(x, &sx, &cx);
+    const double F = x + s1 * (1.0 - cx) - c1 * sx - M;
+    const double F1 = 1.0 + s1 * sx - c1 * cx;
+    const double F2 = s1 * cx + c1 * sx;
+    const double d = -2.0 * F * F1 / (2.0 * F1 * F1 - F * F2);
+    x += d;
+    if (fabs(d) <= 1e-15 * fmax(1.0, fabs(x))) {
+      ok = true;
+      break;
+    }
+  }
+  double sx, cx;
+  sincos(x, &sx, &cx);
+  const double r = a + (r0 - a) * cx + s1 * a * sx;
+  const double f = 1.0 - (a / r0) * (1.0 - cx);
+  const double gg = dt - (x - sx) / n;
+  const double fd = -sqrt(mu * a) * sx / (r * r0);
+  const double gd = 1.0 - (a / r) * (1.0 - cx);
+  for (int i = 0; i < 3; ++i) {
+    out[i] = fma(f, s0[i], gg * s0[3 + i]);
+    out[3 + i] = fma(fd, s0[i], gd * s0[3 + i]);
+  }
+  xio = x;
+  return ok && isfinite(x);
+}
+
+// Battin's form of mu (rb/|rb|^3 - r/|r|^3), r = rb + dr (LY/src/rhs.rs:1071-1104).
+__device__ __forceinline__ void battin(double mu, const double* rb, const double* dr, double& ax, double& ay,
+                                       double& az) {
+  const double rb2 = fma(rb[0], rb[0], fma(rb[1], rb[1], rb[2] * rb[2]));
+  const double rx = rb[0] + dr[0], ry = rb[1] + dr[1], rz = rb[2] + dr[2];
+  const double r2 = fma(rx, rx, fma(ry, ry, rz * rz));
+  if (!(rb2 > 0.0 && r2 > 0.0)) return;
+  const double drb = fma(dr[0], rb[0], fma(dr[1], rb[1], dr[2] * rb[2]));
+  const double dd = fma(dr[0], dr[0], fma(dr[1], dr[1], dr[2] * dr[2]));
+  const double q = (dd + 2.0 * drb) / rb2;
+  const double q1 = 1.0 + q;
+  const double fq = q * (3.0 + q * (3.0 + q)) / (1.0 + q1 * sqrt(q1));
+  const double k = mu / (r2 * sqrt(r2));
+  ax += k * fma(fq, rb[0], -dr[0]);
+  ay += k * fma(fq, rb[1], -dr[1]);
+  az += k * fma(fq, rb[2], -dr[2]);
+}
+
 // Inertial Cowell RHS: uniform z-rotation to the body frame, gravity, rotate
 // back, exponential drag (LY/src/rhs.rs:1119-1163 compute_drag).  The
 // acceleration is computed by wave 0 only and, for lanes with `store`,
@@ -242,18 +327,31 @@ template <int PREC, int G, int S>
 __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
                                            double (*lds)[3][64], double (*sA)[3][64], int slot,
                                            bool store, double bc, double t, const double* y,
-                                           double* acc, uint4* sX, float* sO, int* sFlag, SurLane& sl) {
+                                           double* acc, uint4* sX, float* sO, int* sFlag, SurLane& sl,
+                                           EnckeLane& el) {
+  constexpr bool SUR = (PREC == 2 || PREC == 3);
+  constexpr bool ENCKE = (PREC == 4);
+  // total (inertial) state: y itself, or baseline + deviation (Encke)
+  double yt[6], bst[6];
+  if constexpr (ENCKE) {
+    two_body(p.mu, el.base, t - el.t0, bst, el.x);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) yt[i] = bst[i] + y[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) yt[i] = y[i];
+  }
   const double th = fma(p.model.omega, t, p.model.theta0);
   double s, c;
   sincos(th, &s, &c);
-  const double xb = fma(c, y[0], s * y[1]);
-  const double yb = fma(c, y[1], -(s * y[0]));
-  const double zb = y[2];
+  const double xb = fma(c, yt[0], s * yt[1]);
+  const double yb = fma(c, yt[1], -(s * yt[0]));
+  const double zb = yt[2];
   double gx, gy, gz;
   // surrogate decisions (identical in every wave of the group)
   bool band = true, use_net = false, check = false;
   double rx = 0.0, ry = 0.0, rz = 0.0;
-  if (PREC == 0) {
+  if (PREC == 0 || ENCKE) {
     grav_fp64<G>(p.field, g, xb, yb, zb, gx, gy, gz);
   } else if (PREC == 1) {
     grav_split<G>(p.field, g, xb, yb, zb, gx, gy, gz);
@@ -286,10 +384,10 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
   // would be identical, so they skip the inertial rotation and the drag
   if (G > 1 && g != 0) {
     __syncthreads();
-    if (PREC >= 2) sl.exact = sFlag[lane] != 0;  // wave 0's verdict of this evaluation's check
+    if (SUR) sl.exact = sFlag[lane] != 0;  // wave 0's verdict of this evaluation's check
     return;
   }
-  if (PREC >= 2) {
+  if (SUR) {
     sur_combine(p, sl, store, band, use_net, check, xb, yb, zb, rx, ry, rz, gx, gy, gz);
     if (G > 1) sFlag[lane] = sl.exact ? 1 : 0;
   }
@@ -297,18 +395,28 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
   double ay = fma(s, gx, c * gy);
   double az = gz;
   if (p.model.drag != DRAG_NONE && bc != 0.0) {
-    const double r = sqrt(fma(y[0], y[0], fma(y[1], y[1], y[2] * y[2])));
+    const double r = sqrt(fma(yt[0], yt[0], fma(yt[1], yt[1], yt[2] * yt[2])));
     const double rho = density(p.model.drag, r - p.model.r_atm);
     if (rho > 0.0) {
-      const double vrx = fma(p.model.omega, y[1], y[3]);
-      const double vry = fma(-p.model.omega, y[0], y[4]);
-      const double vrz = y[5];
+      const double vrx = fma(p.model.omega, yt[1], yt[3]);
+      const double vry = fma(-p.model.omega, yt[0], yt[4]);
+      const double vrz = yt[5];
       const double vr = sqrt(fma(vrx, vrx, fma(vry, vry, vrz * vrz)));
       const double k = -0.5 * bc * rho * vr;
       ax = fma(k, vrx, ax);
       ay = fma(k, vry, ay);
       az = fma(k, vrz, az);
     }
+  }
+  if constexpr (ENCKE) {
+    // the baseline carries the central term: add it back, then Battin's
+    // correction of the baseline's two-body acceleration
+    const double r2 = fma(yt[0], yt[0], fma(yt[1], yt[1], yt[2] * yt[2]));
+    const double k = p.mu / (r2 * sqrt(r2));
+    ax = fma(k, yt[0], ax);
+    ay = fma(k, yt[1], ay);
+    az = fma(k, yt[2], az);
+    battin(p.mu, bst, y, ax, ay, az);
   }
   acc[0] = ax;
   acc[1] = ay;
@@ -326,6 +434,8 @@ __global__ void __launch_bounds__(64 * G) __attribute__((amdgpu_waves_per_eu(BQP
 propagate_kernel(const PropParams p) {
   using MT = MethodTraits<METHOD>;
   constexpr int S = MT::S;
+  constexpr bool SUR = (PREC == 2 || PREC == 3);
+  constexpr bool ENCKE = (PREC == 4);
   const bqp_tableau& T = c_tab[METHOD];
   const int lane = threadIdx.x & 63;
   // wave index within the group; readfirstlane tells the compiler it is
@@ -335,10 +445,11 @@ propagate_kernel(const PropParams p) {
   __shared__ double lds[G][3][64];
   __shared__ unsigned int lds_base;
   // surrogate: bf16 features of the wave's 64 lanes (32 k-slots each) and outputs
-  __shared__ uint4 sX[PREC >= 2 ? 64 * 4 : 1];
-  __shared__ float sO[PREC >= 2 ? 64 * 3 : 1];
-  __shared__ int sFlag[PREC >= 2 && G > 1 ? 64 : 1];
+  __shared__ uint4 sX[SUR ? 64 * 4 : 1];
+  __shared__ float sO[SUR ? 64 * 3 : 1];
+  __shared__ int sFlag[SUR && G > 1 ? 64 : 1];
   SurLane sl;
+  EnckeLane el{};
 
   const bool compat = (p.control == CONTROL_COMPAT);
   const bool sampled = (p.t_eval != nullptr);
@@ -372,6 +483,54 @@ propagate_kernel(const PropParams p) {
     if (p.evals) p.evals[row] = evals;
     if (p.rejects) p.rejects[row] = rejects;
     active = false;
+  };
+
+  // Step-size bounds and initial step of an integration over `span`
+  // (solver.rs:385-391 default h0; lightyear_compat.rs:203-213 for compat).
+  auto start_controller = [&](double span) {
+    if (compat) {
+      eps = fmax(p.eps, 1e-12);
+      hmin = 1e-10;
+      hmax = (p.h_max > 0.0 && isfinite(p.h_max)) ? fmax(p.h_max, hmin) : span;
+      h = clampd(fabs(span / 100.0), 1e-3, fmax(hmax, 1e-3));
+    } else {
+      hmax = (p.h_max > 0.0 && isfinite(p.h_max)) ? p.h_max : span;
+      hmin = (p.h_min > 0.0) ? p.h_min : 1e-12;
+      if (hmax < hmin) hmax = hmin;
+      const double h0 = (p.h0 > 0.0) ? p.h0 : ((span <= 60.0) ? span / 2.0 : span / 100.0);
+      h = clampd(h0, hmin, hmax);
+    }
+  };
+
+  // Encke: start a segment at the current time from total state `tot`
+  // (deviation 0, fresh controller), integrator.rs:3418-3563.
+  auto start_segment = [&](const double* tot) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      el.base[i] = tot[i];
+      y[i] = 0.0;
+    }
+    el.t0 = t;
+    el.x = __builtin_nan("");
+    el.seg_end = (tend - t > ENCKE_SEGMENT_S) ? t + ENCKE_SEGMENT_S : tend;
+    start_controller(el.seg_end - t);
+    have_prev = just_rej = have_k1 = false;
+    eprev = 0.0;
+    rej_run = 0;
+    tcomp = 0.0;
+  };
+
+  // Total (inertial) state of the lane at the current time.
+  auto total_state = [&](double* tot) {
+    if constexpr (ENCKE) {
+      double b[6];
+      two_body(p.mu, el.base, t - el.t0, b, el.x);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) tot[i] = b[i] + y[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) tot[i] = y[i];
+    }
   };
 
   while (true) {
@@ -427,20 +586,19 @@ propagate_kernel(const PropParams p) {
             const double span = tend - t;
             if (span == 0.0 || (sampled && sidx >= p.m_eval)) {
               finish(ROW_OK);
+            } else if constexpr (ENCKE) {
+              double tot[6], probe[6], x0 = 0.0;
+#pragma unroll
+              for (int i = 0; i < 6; ++i) tot[i] = y[i];
+              if (!two_body(p.mu, tot, 0.0, probe, x0)) {
+                finish(ROW_INVALID_INPUT);  // the baseline must be an ellipse
+              } else {
+                start_segment(tot);
+                tseg = sampled ? fmin(p.t_eval[sidx], el.seg_end) : el.seg_end;
+              }
             } else {
               tseg = sampled ? p.t_eval[sidx] : tend;
-              if (compat) {
-                eps = fmax(p.eps, 1e-12);
-                hmin = 1e-10;
-                hmax = (p.h_max > 0.0 && isfinite(p.h_max)) ? fmax(p.h_max, hmin) : span;
-                h = clampd(fabs(span / 100.0), 1e-3, fmax(hmax, 1e-3));
-              } else {
-                hmax = (p.h_max > 0.0 && isfinite(p.h_max)) ? p.h_max : span;
-                hmin = (p.h_min > 0.0) ? p.h_min : 1e-12;
-                if (hmax < hmin) hmax = hmin;
-                const double h0 = (p.h0 > 0.0) ? p.h0 : ((span <= 60.0) ? span / 2.0 : span / 100.0);
-                h = clampd(h0, hmin, hmax);
-              }
+              start_controller(span);
             }
           }
         }
@@ -501,8 +659,8 @@ propagate_kernel(const PropParams p) {
         }
       }
       double as[3];
-      if (PREC >= 2) sl.check_now = p.sur_check_every > 0 && (evals % p.sur_check_every) == 0;
-      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, st, bc, t + T.c[s] * hs, ys, as, sX, sO, sFlag, sl);
+      if (SUR) sl.check_now = p.sur_check_every > 0 && (evals % p.sur_check_every) == 0;
+      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, st, bc, t + T.c[s] * hs, ys, as, sX, sO, sFlag, sl, el);
       if (st) ++evals;
     }
     if (needk) have_k1 = true;
@@ -657,9 +815,34 @@ propagate_kernel(const PropParams p) {
       eprev = en;
       have_prev = true;
       just_rej = false;
+      double rmin_tot[6];
+      if (p.model.r_min > 0.0) total_state(rmin_tot);
       if (p.model.r_min > 0.0 &&
-          !(sqrt(fma(y[0], y[0], fma(y[1], y[1], y[2] * y[2]))) >= p.model.r_min)) {
+          !(sqrt(fma(rmin_tot[0], rmin_tot[0], fma(rmin_tot[1], rmin_tot[1], rmin_tot[2] * rmin_tot[2]))) >=
+            p.model.r_min)) {
         finish(ROW_BELOW_MIN_RADIUS);
+      } else if (ENCKE && t == tseg) {
+        double tot[6];
+        total_state(tot);
+        bool done = false;
+        if (sampled) {
+          if (sidx < p.m_eval && t == p.t_eval[sidx]) {
+            for (int i = 0; i < 6; ++i) p.ysamp[((size_t)row * p.m_eval + sidx) * 6 + i] = tot[i];
+            ++sidx;
+          }
+          done = (sidx >= p.m_eval);
+        } else {
+          done = (t == tend);
+        }
+        if (done) {
+#pragma unroll
+          for (int i = 0; i < 6; ++i) y[i] = tot[i];
+          finish(ROW_OK);
+        } else {
+          if (t == el.seg_end) start_segment(tot);  // rebase
+          tseg = sampled ? fmin(p.t_eval[sidx], el.seg_end) : el.seg_end;
+          tcomp = 0.0;
+        }
       } else if (t == tseg) {
         if (sampled) {
           for (int i = 0; i < 6; ++i) p.ysamp[((size_t)row * p.m_eval + sidx) * 6 + i] = y[i];
@@ -696,7 +879,7 @@ propagate_kernel(const PropParams p) {
     // publish the FSAL copy of sA[0] before the next step reads it
     if (MT::FSAL && G > 1) __syncthreads();
   }
-  if (PREC >= 2 && g == 0) {
+  if (SUR && g == 0) {
     unsigned long long v[5] = {sl.net, sl.exact_n, sl.checks, sl.fails, sl.band};
 #pragma unroll
     for (int k = 0; k < 5; ++k) {

@@ -1,0 +1,17 @@
+// Encke form of the persistent propagation kernel (FP64): the deviation from
+// a two-body baseline is integrated with Battin's correction and rebased every
+// 5,400 s, the reference's native formulation.
+#include "prop_forms.hpp"
+
+namespace bqp {
+BQP_FORM_UNIT(pick_encke, upload_encke, 4)
+
+prop_fn pick_encke(int method, int G) {
+  switch (method) {
+    case BQP_METHOD_DOPRI5: return pick_w<BQP_METHOD_DOPRI5, 4>(G);
+    case BQP_METHOD_DOP853: return pick_w<BQP_METHOD_DOP853, 4>(G);
+    case BQP_METHOD_RK87: return pick_w<BQP_METHOD_RK87, 4>(G);
+    default: return nullptr;
+  }
+}
+}  // namespace bqp

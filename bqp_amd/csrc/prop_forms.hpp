@@ -1,7 +1,7 @@
 // Per-form translation units of the persistent propagation kernel.
 //
 // Each precision form (FP64, split, surrogate) instantiates its kernels in a
-// translation unit of its own (prop_fp64.hip, prop_split.hip, prop_sur.hip, prop_sur_fp8.hip)
+// translation unit of its own (prop_fp64.hip, prop_split.hip, prop_sur.hip, prop_sur_fp8.hip, prop_encke.hip)
 // so the forms compile in parallel.  A unit has its own copy of the
 // __constant__ tables, which bqp_create fills through its upload function.
 #pragma once
@@ -20,11 +20,13 @@ prop_fn pick_split(int method, int G);
 prop_fn pick_sur(int method, int prec, int G);  // prec 2 (BF16 network) or 3 (FP8 network)
 prop_fn pick_sur_bf16(int method, int G);
 prop_fn pick_sur_fp8(int method, int G);
+prop_fn pick_encke(int method, int G);
 
 hipError_t upload_fp64(const SecondOrderTableau* so, const double* sect, const float* sectf);
 hipError_t upload_split(const SecondOrderTableau* so, const double* sect, const float* sectf);
 hipError_t upload_sur(const SecondOrderTableau* so, const double* sect, const float* sectf);
 hipError_t upload_sur_fp8(const SecondOrderTableau* so, const double* sect, const float* sectf);
+hipError_t upload_encke(const SecondOrderTableau* so, const double* sect, const float* sectf);
 
 // Body of a form's unit: kernel selection over the four group widths and
 // the constant upload.

@@ -50,12 +50,13 @@ class Propagator:
                  atol_pos: float | None = None, atol_vel: float | None = None, eps: float = 1e-8,
                  h0: float = 0.0, h_min: float = 0.0, h_max: float = 0.0, max_steps: int = 500000,
                  max_rejects: int = 50, omega: float = OMEGA_EARTH, theta0: float = 0.0,
-                 r_atm: float = R_EARTH, r_min: float = 0.0, launch: dict | None = None):
+                 r_atm: float = R_EARTH, r_min: float = 0.0, launch: dict | None = None,
+                 formulation: str = "cowell"):
         self._lib = nat.load()
         self._ctx = C.c_void_p()
         self.field = field
         cfg = self._make_config(method, control, precision, drag, rtol, atol_pos, atol_vel, eps, h0, h_min,
-                                h_max, max_steps, max_rejects, omega, theta0, r_atm, r_min, field)
+                                h_max, max_steps, max_rejects, omega, theta0, r_atm, r_min, field, formulation)
         rc = self._lib.bqp_create(C.byref(cfg), C.byref(self._ctx))
         if rc != 0:
             if rc == nat.E_INVALID:
@@ -70,9 +71,10 @@ class Propagator:
     # ------------------------------------------------------------ plumbing
     @staticmethod
     def _make_config(method, control, precision, drag, rtol, atol_pos, atol_vel, eps, h0, h_min, h_max,
-                     max_steps, max_rejects, omega, theta0, r_atm, r_min, field) -> nat.Config:
+                     max_steps, max_rejects, omega, theta0, r_atm, r_min, field, formulation="cowell") -> nat.Config:
         for name, table, val in (("method", nat.METHODS, method), ("control", nat.CONTROLS, control),
-                                 ("precision", nat.PRECISIONS, precision), ("drag", nat.DRAGS, drag)):
+                                 ("precision", nat.PRECISIONS, precision), ("drag", nat.DRAGS, drag),
+                                 ("formulation", nat.FORMULATIONS, formulation)):
             if val not in table:
                 raise ValueError(f"unknown {name} {val!r}; use one of {sorted(table)}")
         du = field.radius if field is not None else R_EARTH
@@ -83,6 +85,7 @@ class Propagator:
         cfg.control = nat.CONTROLS[control]
         cfg.precision = nat.PRECISIONS[precision]
         cfg.drag = nat.DRAGS[drag]
+        cfg.formulation = nat.FORMULATIONS[formulation]
         cfg.rtol = rtol
         cfg.atol_pos = rtol * du if atol_pos is None else atol_pos
         cfg.atol_vel = rtol * vu if atol_vel is None else atol_vel
@@ -128,7 +131,8 @@ class Propagator:
     def reconfigure(self, **kw):
         """Change integrator/model settings (same keywords as the constructor)."""
         cur = {k: getattr(self.config, k) for k, _ in nat.Config._fields_}
-        names = {"method": nat.METHODS, "control": nat.CONTROLS, "precision": nat.PRECISIONS, "drag": nat.DRAGS}
+        names = {"method": nat.METHODS, "control": nat.CONTROLS, "precision": nat.PRECISIONS, "drag": nat.DRAGS,
+                 "formulation": nat.FORMULATIONS}
         for k, v in kw.items():
             if k in names:
                 if v not in names[k]:

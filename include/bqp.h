@@ -64,6 +64,15 @@ extern "C" {
 
 #define BQP_PRECISION_FP64 0   /* plain FP64 (the correctness oracle form) */
 #define BQP_PRECISION_SPLIT 1  /* degree <= 4 in FP64, higher degrees FP32 + compensated sums */
+/* Equations of motion.  Cowell: the inertial state is integrated.  Encke:
+ * the deviation from a two-body baseline is integrated with Battin's
+ * correction and the baseline is rebased every 5,400 s, each segment with a
+ * fresh controller -- the reference's formulation (LY/src/rhs.rs:3885-4118,
+ * LY/src/integrator.rs:3418-3563); inputs and outputs stay inertial states.
+ * Encke needs precision FP64 and elliptic states. */
+#define BQP_FORMULATION_COWELL 0
+#define BQP_FORMULATION_ENCKE 1
+
 #define BQP_PRECISION_SURROGATE 2 /* degree <= nlow in FP64, the rest from the learned network
                                      (bqp_surrogate_*), exact-field fallback */
 
@@ -83,7 +92,7 @@ typedef struct bqp_config {
   double h_max;        /* <= 0: unlimited (the row's span) */
   int64_t max_steps;   /* <= 0: 500000 */
   int32_t max_rejects; /* <= 0: 50 */
-  int32_t reserved0;
+  int32_t formulation;  /* BQP_FORMULATION_COWELL (0) or BQP_FORMULATION_ENCKE (1) */
   double omega;        /* body-frame rotation rate about +z (rad/s) */
   double theta0;       /* body-frame angle at t = 0 (rad) */
   double r_atm;        /* altitude = |r| - r_atm (m) */

@@ -318,6 +318,132 @@ void orc_rhs(const orc_model* md, double bc, double t, const double y[6], double
   dy[5] = az;
 }
 
+/* ------------------------------------------------------- Encke pieces */
+
+static double orc_field_mu(const orc_field* f) { return f->mu; }
+
+/* Two-body propagation by Lagrange coefficients in the eccentric-anomaly
+ * change x (Battin 1999 §4.5; the reference propagates its baseline the same
+ * way from equinoctial elements, SC/src/lib.rs:2191-2572, 3448):
+ *   n dt = x + s1 (1 - cos x) - c1 sin x,  s1 = (r0.v0)/sqrt(mu a),  c1 = 1 - r0/a
+ *   r = a + (r0 - a) cos x + s1 a sin x
+ *   f = 1 - (a/r0)(1 - cos x),      g = dt - (x - sin x)/n
+ *   fd = -sqrt(mu a) sin x/(r r0),  gd = 1 - (a/r)(1 - cos x)
+ * solved by Halley's method (the reference's Halley-type longitude solve,
+ * tolerance 1e-12, is restated at 1e-15 rad). */
+int orc_two_body(double mu, const double s0[6], double dt, double out[6], double* xio) {
+  const double r0 = sqrt(fma(s0[0], s0[0], fma(s0[1], s0[1], s0[2] * s0[2])));
+  const double v2 = fma(s0[3], s0[3], fma(s0[4], s0[4], s0[5] * s0[5]));
+  const double rv = fma(s0[0], s0[3], fma(s0[1], s0[4], s0[2] * s0[5]));
+  const double ainv = 2.0 / r0 - v2 / mu;
+  if (!(ainv > 0.0) || !isfinite(ainv)) return ORC_INVALID_INPUT;
+  const double a = 1.0 / ainv;
+  const double sqa = sqrt(a);
+  const double n = sqrt(mu * ainv * ainv * ainv);
+  const double s1 = rv / (sqrt(mu) * sqa);
+  const double c1 = 1.0 - r0 * ainv;
+  const double M = n * dt;
+  double x = (xio && isfinite(*xio)) ? *xio : M;
+  int ok = 0;
+  for (int it = 0; it < 50; ++it) {
+    const double sx = sin(x), cx = cos(x);
+    const double F = x + s1 * (1.0 - cx) - c1 * sx - M;
+    const double F1 = 1.0 + s1 * sx - c1 * cx;
+    const double F2 = s1 * cx + c1 * sx;
+    const double d = -2.0 * F * F1 / (2.0 * F1 * F1 - F * F2);
+    x += d;
+    if (fabs(d) <= 1e-15 * fmax(1.0, fabs(x))) {
+      ok = 1;
+      break;
+    }
+  }
+  if (!ok || !isfinite(x)) return ORC_INVALID_INPUT;
+  const double sx = sin(x), cx = cos(x);
+  const double r = a + (r0 - a) * cx + s1 * a * sx;
+  const double f = 1.0 - (a / r0) * (1.0 - cx);
+  const double g = dt - (x - sx) / n;
+  const double fd = -sqrt(mu * a) * sx / (r * r0);
+  const double gd = 1.0 - (a / r) * (1.0 - cx);
+  for (int i = 0; i < 3; ++i) {
+    out[i] = fma(f, s0[i], g * s0[3 + i]);
+    out[3 + i] = fma(fd, s0[i], gd * s0[3 + i]);
+  }
+  if (xio) *xio = x;
+  return ORC_OK;
+}
+
+/* Battin's Encke correction (LY/src/rhs.rs:1071-1104, Battin 1999 §9.3):
+ * mu (rb/|rb|^3 - r/|r|^3) = mu/|r|^3 (f(q) rb - dr),
+ * q = dr.(dr + 2 rb)/|rb|^2,  f(q) = q (3 + 3q + q^2)/(1 + (1+q)^{3/2}). */
+void orc_battin(double mu, const double rb[3], const double dr[3], double acc[3]) {
+  const double rb2 = fma(rb[0], rb[0], fma(rb[1], rb[1], rb[2] * rb[2]));
+  double r[3];
+  for (int i = 0; i < 3; ++i) r[i] = rb[i] + dr[i];
+  const double r2 = fma(r[0], r[0], fma(r[1], r[1], r[2] * r[2]));
+  if (!(rb2 > 0.0 && r2 > 0.0)) return;
+  const double drb = fma(dr[0], rb[0], fma(dr[1], rb[1], dr[2] * rb[2]));
+  const double dd = fma(dr[0], dr[0], fma(dr[1], dr[1], dr[2] * dr[2]));
+  const double q = (dd + 2.0 * drb) / rb2;
+  const double q1 = 1.0 + q;
+  const double fq = q * (3.0 + q * (3.0 + q)) / (1.0 + q1 * sqrt(q1));
+  const double k = mu / (r2 * sqrt(r2));
+  for (int i = 0; i < 3; ++i) acc[i] += k * fma(fq, rb[i], -dr[i]);
+}
+
+/* Right-hand side of either formulation.  Encke: the state is the deviation
+ * d from the two-body baseline started at base (time t0); the total state is
+ * baseline + d; the acceleration is the full field plus mu r/|r|^3 (the
+ * baseline carries the central term), drag on the total state, plus Battin's
+ * correction (LY/src/rhs.rs:3885-4118). */
+typedef struct rhs_ctx {
+  const orc_model* md;
+  double bc;
+  double base[6];
+  double t0;
+  double x;  /* warm start of the baseline's Kepler solve */
+  int bad;   /* baseline failed */
+} rhs_ctx;
+
+static void eval_rhs(rhs_ctx* c, double t, const double y[6], double dy[6]) {
+  if (c->md->formulation != ORC_ENCKE) {
+    orc_rhs(c->md, c->bc, t, y, dy);
+    return;
+  }
+  double b[6], tot[6];
+  if (orc_two_body(orc_field_mu(c->md->field), c->base, t - c->t0, b, &c->x) != ORC_OK) {
+    c->bad = 1;
+    for (int i = 0; i < 6; ++i) dy[i] = NAN;
+    return;
+  }
+  for (int i = 0; i < 6; ++i) tot[i] = b[i] + y[i];
+  double a[6];
+  orc_rhs(c->md, c->bc, t, tot, a);
+  const double mu = orc_field_mu(c->md->field);
+  const double r2 = fma(tot[0], tot[0], fma(tot[1], tot[1], tot[2] * tot[2]));
+  const double k = mu / (r2 * sqrt(r2));
+  double acc[3];
+  for (int i = 0; i < 3; ++i) acc[i] = fma(k, tot[i], a[3 + i]);
+  orc_battin(mu, b, y, acc);
+  dy[0] = y[3];
+  dy[1] = y[4];
+  dy[2] = y[5];
+  dy[3] = acc[0];
+  dy[4] = acc[1];
+  dy[5] = acc[2];
+}
+
+/* Total (inertial) state of the context at t. */
+static int total_state(rhs_ctx* c, double t, const double y[6], double out[6]) {
+  if (c->md->formulation != ORC_ENCKE) {
+    memcpy(out, y, 6 * sizeof(double));
+    return ORC_OK;
+  }
+  double b[6];
+  const int rc = orc_two_body(orc_field_mu(c->md->field), c->base, t - c->t0, b, &c->x);
+  for (int i = 0; i < 6; ++i) out[i] = b[i] + y[i];
+  return rc;
+}
+
 /* -------------------------------------------------------------- tableaus */
 
 static const bqp_tableau T_DOPRI5 = BQP_DOPRI5_TABLEAU;
@@ -353,12 +479,12 @@ static double clampd(double v, double lo, double hi) { return v < lo ? lo : (v >
 /* One explicit RK step (LY/src/odesolve/solver.rs:1965-2173 rk_step):
  * stage sums by fma-axpy with h*a_ij (zero coefficients skipped), solution by
  * fma-axpy with h*b_j, error sums with Kahan compensation. */
-static void rk_step(const orc_model* md, double bc, const bqp_tableau* T, double t,
+static void rk_step(rhs_ctx* rc, const bqp_tableau* T, double t,
                     const double y[6], double h, int reuse_k1, double k[BQP_MAX_STAGES][6],
                     double yn[6], double err[6], double err3[6], int64_t* evals) {
   double yt[6];
   if (!reuse_k1) {
-    orc_rhs(md, bc, t, y, k[0]);
+    eval_rhs(rc, t, y, k[0]);
     ++*evals;
   }
   for (int s = 1; s < T->stages; ++s) {
@@ -368,7 +494,7 @@ static void rk_step(const orc_model* md, double bc, const bqp_tableau* T, double
       if (sc == 0.0) continue;
       for (int i = 0; i < 6; ++i) yt[i] = fma(k[j][i], sc, yt[i]);
     }
-    orc_rhs(md, bc, t + T->c[s] * h, yt, k[s]);
+    eval_rhs(rc, t + T->c[s] * h, yt, k[s]);
     ++*evals;
   }
   double comp[6] = {0}, comp3[6] = {0};
@@ -402,29 +528,16 @@ static void rk_step(const orc_model* md, double bc, const bqp_tableau* T, double
 
 static double radius_of(const double* y) { return sqrt(fma(y[0], y[0], fma(y[1], y[1], y[2] * y[2]))); }
 
-/* Adaptive loop: LY/src/odesolve/solver.rs:1099-1925 integrate_internal with
- * the PI controller macros :445-529 (ORC_CONTROL_SCALED / _ABSOLUTE), or
- * LY/src/odesolve/lightyear_compat.rs:100-600 (ORC_CONTROL_COMPAT). */
-int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const double y0[6],
-                  double t0, double tf, double yf[6], orc_stats* st) {
-  orc_stats loc = {0, 0, 0, ORC_OK};
-  const bqp_tableau* T = tableau_of(cfg->method);
-  double y[6];
-  memcpy(y, y0, sizeof(y));
-  if (!T || !finite6(y0) || !isfinite(t0) || !isfinite(tf) || !(tf >= t0)) {
-    loc.status = ORC_INVALID_INPUT;
-    goto fail;
-  }
-  if (md->r_min > 0.0 && !(radius_of(y) >= md->r_min)) {
-    loc.status = ORC_BELOW_MIN_RADIUS;
-    goto fail;
-  }
-  const double span = tf - t0;
-  if (span == 0.0) {
-    memcpy(yf, y, sizeof(y));
-    if (st) *st = loc;
-    return ORC_OK;
-  }
+/* One adaptive integration of y over [ta, tb] with a fresh controller:
+ * LY/src/odesolve/solver.rs:1099-1925 integrate_internal with the PI
+ * controller macros :445-529 (ORC_CONTROL_SCALED / _ABSOLUTE), or
+ * LY/src/odesolve/lightyear_compat.rs:100-600 (ORC_CONTROL_COMPAT).  The step
+ * budget is per call, as the reference's per-segment solver call. */
+static int integrate_segment(rhs_ctx* rc, const orc_integ* cfg, const bqp_tableau* T, double y[6],
+                             double ta, double tb, orc_stats* loc) {
+  const double span = tb - ta;
+  if (span == 0.0) return ORC_OK;
+  int64_t seg_steps = 0, seg_rejects = 0;
   const int compat = (cfg->control == ORC_CONTROL_COMPAT);
   const double q = compat ? 5.0 : (double)T->order_err + 1.0;
   const double inv_q = 1.0 / q;
@@ -446,35 +559,36 @@ int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const do
   double k[BQP_MAX_STAGES][6], yn[6], err[6], err3[6], k1c[6];
   int have_k1 = 0;
   if (compat) { /* the compat loop evaluates dy once up front and reuses it */
-    orc_rhs(md, bc, t0, y, k1c);
-    loc.evals++;
+    eval_rhs(rc, ta, y, k1c);
+    loc->evals++;
     have_k1 = 1;
   }
-  double t = t0, t_comp = 0.0, err_prev = 0.0;
+  double t = ta, t_comp = 0.0, err_prev = 0.0;
   int have_prev = 0, just_rej = 0, rejects = 0;
   const int blend3 = T->has_e3 && (cfg->control == ORC_CONTROL_SCALED ||
                                    (cfg->control == ORC_CONTROL_ABSOLUTE && eps >= 1e-6));
-  while (t != tf && tf - t > 0.0) {
-    if ((compat ? loc.steps + loc.rejects : loc.steps) >= max_steps) {
-      loc.status = ORC_MAX_STEPS;
-      goto fail;
+  while (t != tb && tb - t > 0.0) {
+    if ((compat ? seg_steps + seg_rejects : seg_steps) >= max_steps) {
+      loc->status = ORC_MAX_STEPS;
+      return loc->status;
     }
-    const double rem = tf - t;
+    const double rem = tb - t;
     double hs = (h < rem) ? h : rem;
     const int lands = (hs == rem);
     if (!compat && hs < h_min && !lands) {
-      loc.status = ORC_STEP_UNDERFLOW;
-      goto fail;
+      loc->status = ORC_STEP_UNDERFLOW;
+      return loc->status;
     }
     if (have_k1) memcpy(k[0], k1c, sizeof(k1c));
-    rk_step(md, bc, T, t, y, hs, have_k1, k, yn, err, err3, &loc.evals);
+    rk_step(rc, T, t, y, hs, have_k1, k, yn, err, err3, &loc->evals);
     if (!finite6(yn) || !finite6(err)) {
       ++rejects;
-      ++loc.rejects;
+      ++loc->rejects;
+      ++seg_rejects;
       if (!compat) have_k1 = 0;
       if (rejects > max_rejects) {
-        loc.status = ORC_NON_FINITE;
-        goto fail;
+        loc->status = ORC_NON_FINITE;
+        return loc->status;
       }
       h = clampd(hs * 0.1, h_min, h_max);
       continue;
@@ -512,11 +626,12 @@ int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const do
     }
     if (!isfinite(en)) {
       ++rejects;
-      ++loc.rejects;
+      ++loc->rejects;
+      ++seg_rejects;
       if (!compat) have_k1 = 0;
       if (rejects > max_rejects) {
-        loc.status = ORC_NON_FINITE;
-        goto fail;
+        loc->status = ORC_NON_FINITE;
+        return loc->status;
       }
       h = clampd(hs * 0.1, h_min, h_max);
       continue;
@@ -524,13 +639,14 @@ int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const do
     if (en <= thresh || hs <= h_min) {
       double tn;
       if (lands) {
-        tn = tf;
+        tn = tb;
       } else {
         const double ky = hs - t_comp;
         tn = t + ky;
         t_comp = (tn - t) - ky;
       }
-      ++loc.steps;
+      ++loc->steps;
+      ++seg_steps;
       rejects = 0;
       if (T->fsal) {
         memcpy(k1c, k[T->stages - 1], sizeof(k1c));
@@ -539,10 +655,13 @@ int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const do
         have_k1 = 0;
       }
       t = tn;
-      memcpy(y, yn, sizeof(y));
-      if (md->r_min > 0.0 && !(radius_of(y) >= md->r_min)) {
-        loc.status = ORC_BELOW_MIN_RADIUS;
-        goto fail;
+      memcpy(y, yn, 6 * sizeof(double));
+      if (rc->md->r_min > 0.0) {
+        double tot[6];
+        if (total_state(rc, t, y, tot) != ORC_OK || !(radius_of(tot) >= rc->md->r_min)) {
+          loc->status = ORC_BELOW_MIN_RADIUS;
+          return loc->status;
+        }
       }
       const double eff = (en > 0.0) ? en : thresh * 0.1;
       double fac, maxg;
@@ -570,11 +689,12 @@ int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const do
       just_rej = 0;
     } else {
       ++rejects;
-      ++loc.rejects;
+      ++loc->rejects;
+      ++seg_rejects;
       if (!compat) have_k1 = 0;
       if (rejects > max_rejects) {
-        loc.status = ORC_MAX_REJECTS;
-        goto fail;
+        loc->status = ORC_MAX_REJECTS;
+        return loc->status;
       }
       double fac;
       if (cfg->control == ORC_CONTROL_SCALED)
@@ -588,6 +708,59 @@ int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const do
         have_prev = 1;
       }
       just_rej = 1;
+    }
+  }
+  (void)seg_rejects;
+  return ORC_OK;
+}
+
+/* Propagation of one row.  Cowell: one segment.  Encke (ORC_ENCKE): the
+ * segment driver of LY/src/integrator.rs:3418-3563 -- segments of at most
+ * ORC_ENCKE_SEGMENT_S (:236 MAX_RECTIFICATION_SEGMENT_S, bounded_segment_end
+ * :953-961), each integrating the deviation from 0 on a two-body baseline
+ * started from the current total state, then rebased.  (The reference's
+ * 10 km deviation test only decides how the last segment's result is
+ * expressed, :941-951, so the total state is the same.) */
+int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const double y0[6],
+                  double t0, double tf, double yf[6], orc_stats* st) {
+  orc_stats loc = {0, 0, 0, ORC_OK};
+  const bqp_tableau* T = tableau_of(cfg->method);
+  double y[6];
+  memcpy(y, y0, sizeof(y));
+  if (!T || !finite6(y0) || !isfinite(t0) || !isfinite(tf) || !(tf >= t0)) {
+    loc.status = ORC_INVALID_INPUT;
+    goto fail;
+  }
+  if (md->r_min > 0.0 && !(radius_of(y) >= md->r_min)) {
+    loc.status = ORC_BELOW_MIN_RADIUS;
+    goto fail;
+  }
+  rhs_ctx rc;
+  memset(&rc, 0, sizeof(rc));
+  rc.md = md;
+  rc.bc = bc;
+  if (md->formulation != ORC_ENCKE) {
+    if (integrate_segment(&rc, cfg, T, y, t0, tf, &loc) != ORC_OK) goto fail;
+  } else {
+    double ts = t0;
+    while (1) {
+      const double te = (tf - ts > ORC_ENCKE_SEGMENT_S) ? ts + ORC_ENCKE_SEGMENT_S : tf;
+      memcpy(rc.base, y, sizeof(y));
+      rc.t0 = ts;
+      rc.x = NAN;
+      double d[6] = {0, 0, 0, 0, 0, 0};
+      double probe[6];
+      if (orc_two_body(md->field->mu, rc.base, 0.0, probe, NULL) != ORC_OK) {
+        loc.status = ORC_INVALID_INPUT;  /* baseline is not an ellipse */
+        goto fail;
+      }
+      if (integrate_segment(&rc, cfg, T, d, ts, te, &loc) != ORC_OK) goto fail;
+      if (total_state(&rc, te, d, y) != ORC_OK) {
+        loc.status = ORC_NON_FINITE;
+        goto fail;
+      }
+      ts = te;
+      if (te == tf) break;
     }
   }
   memcpy(yf, y, sizeof(y));

@@ -79,7 +79,13 @@ typedef struct orc_model {
   int drag;         /* ORC_DRAG_* */
   double r_atm;     /* radius subtracted from |r| to get altitude, m */
   double r_min;     /* minimum admissible radius, m (0 disables) */
+  int formulation;  /* ORC_COWELL or ORC_ENCKE */
 } orc_model;
+
+/* Equations of motion: Cowell (inertial state) or Encke (deviation from a
+ * two-body baseline, Battin's correction, rebased every ORC_ENCKE_SEGMENT_S). */
+enum { ORC_COWELL = 0, ORC_ENCKE = 1 };
+#define ORC_ENCKE_SEGMENT_S 5400.0
 
 typedef struct orc_integ {
   int method;       /* bqp_method_id */
@@ -114,6 +120,15 @@ int orc_propagate_batch(const orc_model* m, const orc_integ* cfg, const double* 
                         int threads);
 
 void orc_kep2eci(double mu, const double kep[6], double eci[6]);
+
+/* Two-body (Keplerian) state dt seconds after the elliptic state s0 (m, m/s).
+ * *x is the eccentric-anomaly change: on input a starting guess (or NaN),
+ * on output the solution.  Returns ORC_OK or ORC_INVALID_INPUT (not an
+ * ellipse, or no convergence). */
+int orc_two_body(double mu, const double s0[6], double dt, double out[6], double* x);
+
+/* Battin's form of mu (rb/|rb|^3 - r/|r|^3), r = rb + dr, added to acc. */
+void orc_battin(double mu, const double rb[3], const double dr[3], double acc[3]);
 int orc_tableau(int method, bqp_tableau* out);
 
 #ifdef __cplusplus

@@ -39,6 +39,7 @@ class Model(C.Structure):
         ("drag", C.c_int),
         ("r_atm", C.c_double),
         ("r_min", C.c_double),
+        ("formulation", C.c_int),
     ]
 
 
@@ -96,6 +97,8 @@ def lib():
                                           C.c_int64, _dp, _i32p, _i64p, _i64p, C.c_int]
         L.orc_kep2eci.argtypes = [C.c_double, _dp, _dp]
         L.orc_tableau.argtypes = [C.c_int, C.c_void_p]
+        L.orc_two_body.argtypes = [C.c_double, _dp, C.c_double, _dp, _dp]
+        L.orc_battin.argtypes = [C.c_double, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -137,9 +140,12 @@ def density(model: str, h_m: float) -> float:
     return lib().orc_density(DRAG[model], float(h_m))
 
 
+FORMULATION = {"cowell": 0, "encke": 1}
+
+
 def make_model(field: Field, omega=7.292115e-5, theta0=0.0, drag="none", r_atm=6378137.0,
-               r_min=0.0) -> Model:
-    m = Model(field.ptr, omega, theta0, DRAG[drag], r_atm, r_min)
+               r_min=0.0, formulation="cowell") -> Model:
+    m = Model(field.ptr, omega, theta0, DRAG[drag], r_atm, r_min, FORMULATION[formulation])
     m._field_ref = field  # the C field must outlive every model that points at it
     return m
 
@@ -199,6 +205,25 @@ def kep2eci(mu, kep):
     for i in range(kep.shape[0]):
         lib().orc_kep2eci(float(mu), _p(kep[i]), _p(out[i]))
     return out
+
+
+def two_body(mu, s0, dt, x=None):
+    """Two-body state dt after s0; returns (state, eccentric-anomaly change)."""
+    s0 = np.ascontiguousarray(s0, dtype=np.float64)
+    out = np.empty(6)
+    xx = np.array([np.nan if x is None else float(x)])
+    rc = lib().orc_two_body(float(mu), _p(s0), float(dt), _p(out), _p(xx))
+    if rc != 0:
+        raise ValueError("baseline is not an ellipse or the Kepler solve did not converge")
+    return out, float(xx[0])
+
+
+def battin(mu, rb, dr):
+    rb = np.ascontiguousarray(rb, dtype=np.float64)
+    dr = np.ascontiguousarray(dr, dtype=np.float64)
+    acc = np.zeros(3)
+    lib().orc_battin(float(mu), _p(rb), _p(dr), _p(acc))
+    return acc
 
 
 def tableau(method: str) -> dict:
