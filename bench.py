@@ -8,14 +8,16 @@ Workload (one "step" = one pass of the hot path over the whole batch):
 Inputs are resident in HBM before the timed region; the timed region is the
 device call (cost-sort + persistent propagation kernel + row statistics).
 
-`value` is the plain FP64 form.  The split-precision form and the learned
+`value` is the plain FP64 form.  The split-precision form, the learned
 surrogate (trained on the device first, untimed; BF16 network, exact
-fallback on a failed sampled check) run on the same inputs after it; their
-rates and final-position errors against FP64 are reported under "forms".  The CPU baseline is the project's FP64 C
+fallback on a failed sampled check) and the Encke formulation (the
+reference's own: deviation from a two-body baseline rebased every 5,400 s,
+absolute error control) run on the same inputs after it; their rates and
+final-position differences from FP64 are reported under "forms".  The CPU baseline is the project's FP64 C
 restatement (oracle/) on a bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N]
-       [--days D] [--forms fp64,split,surrogate] [--no-cpu]
+       [--days D] [--forms fp64,split,surrogate,encke] [--no-cpu]
        [--workload w1|w2|w3]   (w2: ragged batch; w3: surrogate study)
 """
 from __future__ import annotations
@@ -304,7 +306,7 @@ def run_population(args) -> dict:
     w = inputs.workload("w1", n=pop, seed=7)
     field = bqp_amd.kaula(50)
     r_target = 6378137.0 + 800e3
-    forms = [f for f in args.forms.split(",") if f] if args.forms != "fp64,split,surrogate" else \
+    forms = [f for f in args.forms.split(",") if f] if args.forms != "fp64,split,surrogate,encke" else \
         ["fp64", "surrogate_bf16", "surrogate_fp8"]
     out, finals, sur = {}, {}, None
     prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado")
@@ -410,7 +412,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--rows", type=int, default=65536)
     ap.add_argument("--days", type=float, default=7.0)
-    ap.add_argument("--forms", default="fp64,split,surrogate")
+    ap.add_argument("--forms", default="fp64,split,surrogate,encke")
+    ap.add_argument("--encke-eps", type=float, default=1e-6,
+                    help="encke form: absolute error bound on the deviation (m, m/s) per step")
     ap.add_argument("--sur-rows", type=int, default=10_000_000, help="surrogate training samples")
     ap.add_argument("--sur-steps", type=int, default=3000, help="surrogate Adam steps (batch 8192)")
     ap.add_argument("--sur-check", type=int, default=100, help="surrogate: check every k-th evaluation")
@@ -464,8 +468,12 @@ def main():
     finals = {}
     sur_report = None
     for form in forms:
-        prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado",
-                                  precision="surrogate" if form == "surrogate" else form)
+        if form == "encke":  # the reference's formulation: deviation from a rebased two-body baseline
+            prop = bqp_amd.Propagator(field, method="rk87", control="absolute", eps=args.encke_eps,
+                                      drag="vallado", formulation="encke")
+        else:
+            prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado",
+                                      precision="surrogate" if form == "surrogate" else form)
         if args.launch:
             prop.set_launch(**json.loads(args.launch))
         if form == "surrogate":
@@ -532,6 +540,12 @@ def main():
                 "achieved_tflops": results[form]["achieved_tflops"],
                 "kernel_ms": results[form]["kernel_ms"],
             }
+            if form == "encke":
+                forms_out[form].update({
+                    "formulation": "Encke deviation on a two-body baseline, Battin correction, rebased every "
+                                   "5,400 s (fresh controller per segment), absolute error control",
+                    "eps": args.encke_eps, "rhs_evals": results[form]["rhs_evals"],
+                    "steps": results[form]["steps"]})
             if form == "surrogate":
                 forms_out[form].update({
                     "network": "16 features -> 4 x 256 SiLU -> 3, BF16 MFMA, FP32 accumulate, residual above deg 4",
