@@ -225,6 +225,21 @@ __device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, d
   az = sz + tz;
 }
 
+// FP32 recursion step of the split form ((a, b) entries); the new value
+// overwrites the n-2 register as in BQP_COL_STEP.
+#define BQP_COL_STEP_F(E, VOUT, VPREV, WOUT, WPREV) \
+  {                                                 \
+    const float A_ = (E).a * ztf, B_ = (E).b * rhof; \
+    VOUT = fmaf(A_, VPREV, -(B_ * VOUT));           \
+    WOUT = fmaf(A_, WPREV, -(B_ * WOUT));           \
+    px = fmaf((E).xv, VOUT, px);                    \
+    px = fmaf((E).xw, WOUT, px);                    \
+    py = fmaf((E).yv, VOUT, py);                    \
+    py = fmaf((E).yw, WOUT, py);                    \
+    pz = fmaf((E).zv, VOUT, pz);                    \
+    pz = fmaf((E).zw, WOUT, pz);                    \
+  }
+
 // Split-precision gravity: entries with n <= SPLIT_N (degree <= 4) in FP64,
 // the rest in FP32.  Each FP32 column sum is short (<= N+1 terms) and is
 // folded into an FP32 Neumaier-compensated accumulator, whose value and
@@ -339,22 +354,30 @@ __device__ __forceinline__ void grav_split(const FieldView& f, int g, double x, 
       v2f = 0.0f;
       w2f = 0.0f;
     }
-    // FP32 tail of the column
+    // FP32 tail of the column, four entries per scalar-load batch
     float px = 0.0f, py = 0.0f, pz = 0.0f;
-    for (; n <= top; ++n, ++e, ++ef) {
-      const float A = ef->a * ztf, B = ef->b * rhof;
-      const float v = fmaf(A, v1f, -(B * v2f));
-      const float w = fmaf(A, w1f, -(B * w2f));
-      px = fmaf(ef->xv, v, px);
-      px = fmaf(ef->xw, w, px);
-      py = fmaf(ef->yv, v, py);
-      py = fmaf(ef->yw, w, py);
-      pz = fmaf(ef->zv, v, pz);
-      pz = fmaf(ef->zw, w, pz);
-      v2f = v1f;
-      v1f = v;
-      w2f = w1f;
-      w1f = w;
+    {
+      int kk = top + 1 - n;
+      CEntryF r = ef;
+      for (; kk >= 4; kk -= 4, r += 4) {
+        const GravEntryF f0 = r[0], f1 = r[1], f2 = r[2], f3 = r[3];
+        BQP_COL_STEP_F(f0, v2f, v1f, w2f, w1f);
+        BQP_COL_STEP_F(f1, v1f, v2f, w1f, w2f);
+        BQP_COL_STEP_F(f2, v2f, v1f, w2f, w1f);
+        BQP_COL_STEP_F(f3, v1f, v2f, w1f, w2f);
+      }
+      for (; kk >= 2; kk -= 2, r += 2) {
+        const GravEntryF f0 = r[0], f1 = r[1];
+        BQP_COL_STEP_F(f0, v2f, v1f, w2f, w1f);
+        BQP_COL_STEP_F(f1, v1f, v2f, w1f, w2f);
+      }
+      if (kk) {
+        const GravEntryF f0 = r[0];
+        BQP_COL_STEP_F(f0, v2f, v1f, w2f, w1f);
+      }
+      const int used = top + 1 - n;
+      e += used;
+      ef += used;
     }
     comp_add(cx, px);
     comp_add(cy, py);
