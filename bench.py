@@ -459,6 +459,7 @@ def main():
     y0 = DeviceArray.from_numpy(w["y0"])
     t0 = DeviceArray.from_numpy(np.zeros(n))
     tf = DeviceArray.from_numpy(np.full(n, tf_val))
+    tf_short = DeviceArray.from_numpy(np.full(n, min(tf_val, 3600.0)))
     bc = DeviceArray.from_numpy(w["bc"])
     orbit_days = n * args.days
     fstage = stage_flops("rk87")
@@ -485,8 +486,12 @@ def main():
         status = DeviceArray((n,), np.int32)
         steps_t = DeviceArray((n,), np.int32)
         evals_t = DeviceArray((n,), np.int32)
-        for _ in range(args.warmup if form == forms[0] else 0):
-            prop.propagate_batch_final_dev(y0, t0, tf, bc, yf, status, steps_t, evals_t)
+        if form == forms[0]:
+            for _ in range(args.warmup):
+                prop.propagate_batch_final_dev(y0, t0, tf, bc, yf, status, steps_t, evals_t)
+        elif args.warmup:
+            # later forms: a one-hour launch brings clocks and caches up without a full extra pass
+            prop.propagate_batch_final_dev(y0, t0, tf_short, bc, yf, status, steps_t, evals_t)
         synchronize()
         wall = 0.0
         kern_ms = 0.0

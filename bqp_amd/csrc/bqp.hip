@@ -279,6 +279,130 @@ double grav_flops(int N, int jmax) {
   return f;
 }
 
+int grid_for(bqp_ctx* c, prop_fn kern, int G, int64_t n, int* blocks) {
+  const int64_t groups = (n + 63) / 64;
+  int b = c->launch.blocks;
+  if (b <= 0) {
+    int per_cu = 0;
+    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kern, 64 * G, 0));
+    if (per_cu < 1) per_cu = 1;
+    b = (int)std::min<int64_t>(groups, (int64_t)per_cu * c->cus);
+  }
+  *blocks = (int)std::max<int64_t>(1, std::min<int64_t>(b, groups));
+  return BQP_OK;
+}
+
+// Checked surrogate propagation with the first check made up front: the
+// rows whose initial state fails it run the plain FP64 kernel (the same
+// arithmetic as the surrogate kernel's exact path, so the same results), the
+// others the surrogate kernel.  Both groups keep the longest-first order.
+int run_surrogate_partitioned(bqp_ctx* c, PropParams p, int64_t n, int kprec, int G, prop_fn sur_kern,
+                              bqp_stats* stats) {
+  const bqp_config& cfg = c->cfg;
+  prop_fn exact_kern = pick_kernel(cfg.method, 0, G);
+  if (!exact_kern) return fail(c, BQP_E_INVALID, "waves_per_group must be 1, 2, 4 or 8");
+  int rc = ensure(c, &c->d_part, &c->part_bytes, (size_t)(3 * n + 64) * sizeof(int));
+  if (rc) return rc;
+  int* flags = (int*)c->d_part;
+  int* exact_rows = flags + n;
+  int* sur_rows = exact_rows + n;
+  int* counts = sur_rows + n;
+  // longest-first hand-out order over all rows, then a stable split of it
+  const bool sort = c->launch.sort_by_cost != 0 && p.t_eval == nullptr && n > 64;
+  int32_t* order = nullptr;
+  if (sort) {
+    const size_t need = (size_t)SORT_BUCKETS * sizeof(unsigned int) + (size_t)n * sizeof(int32_t) +
+                        (size_t)n * sizeof(unsigned short) + 512;
+    rc = ensure(c, &c->d_sort, &c->sort_bytes, need);
+    if (rc) return rc;
+    char* base = (char*)c->d_sort;
+    unsigned int* hist = (unsigned int*)base;
+    order = (int32_t*)(base + SORT_BUCKETS * sizeof(unsigned int));
+    unsigned short* bucket = (unsigned short*)(base + SORT_BUCKETS * sizeof(unsigned int) + n * sizeof(int32_t));
+    const unsigned nbs = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(sort_clear_kernel, dim3(1), dim3(1024), 0, c->stream, hist);
+    hipLaunchKernelGGL(sort_hist_kernel, dim3(nbs), dim3(256), 0, c->stream, p.y0, p.t0, p.tf, c->mu, (int)n,
+                       hist, bucket);
+    hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(1024), 0, c->stream, hist);
+    hipLaunchKernelGGL(sort_scatter_kernel, dim3(nbs), dim3(256), 0, c->stream, bucket, (int)n, hist, order);
+  }
+  const unsigned nb = (unsigned)((n + 63) / 64);
+  if (kprec == 3)
+    hipLaunchKernelGGL(sur_precheck_kernel<3>, dim3(nb), dim3(64), 0, c->stream, p, (int)n, flags);
+  else
+    hipLaunchKernelGGL(sur_precheck_kernel<2>, dim3(nb), dim3(64), 0, c->stream, p, (int)n, flags);
+  hipLaunchKernelGGL(partition_kernel, dim3(1), dim3(1024), 0, c->stream, flags, order, (int)n, exact_rows,
+                     sur_rows, counts);
+  HIP_TRY(c, hipGetLastError());
+  int cnt[2] = {0, 0};
+  HIP_TRY(c, hipMemcpyAsync(cnt, counts, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  int be = 1, bs = 1;
+  if (cnt[0] > 0 && (rc = grid_for(c, exact_kern, G, cnt[0], &be))) return rc;
+  if (cnt[1] > 0 && (rc = grid_for(c, sur_kern, G, cnt[1], &bs))) return rc;
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, p.sur_totals, 5);
+  HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
+  if (cnt[0] > 0) {
+    PropParams q = p;
+    q.order = exact_rows;
+    q.n = cnt[0];
+    q.sur_totals = nullptr;
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
+    hipLaunchKernelGGL(exact_kern, dim3(be), dim3(64 * G), 0, c->stream, q);
+  }
+  if (cnt[1] > 0) {
+    PropParams q = p;
+    q.order = sur_rows;
+    q.n = cnt[1];
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
+    hipLaunchKernelGGL(sur_kern, dim3(bs), dim3(64 * G), 0, c->stream, q);
+  }
+  HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_totals, 5);
+  hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, c->stream, c->d_first, 0x7fffffff);
+  hipLaunchKernelGGL(row_stats_kernel, dim3((unsigned)std::min<int64_t>(1024, (n + 255) / 256)), dim3(256), 0,
+                     c->stream, p.status, p.steps, p.evals, p.rejects, (int)n, c->d_totals);
+  hipLaunchKernelGGL(first_failed_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, p.status,
+                     (int)n, c->d_first);
+  HIP_TRY(c, hipGetLastError());
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  unsigned long long tot[5] = {0, 0, 0, 0, 0}, st[5] = {0, 0, 0, 0, 0};
+  int first = 0x7fffffff;
+  HIP_TRY(c, hipMemcpy(tot, c->d_totals, sizeof(tot), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(&first, c->d_first, sizeof(int), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(st, c->mlp->d_sur_totals, sizeof(st), hipMemcpyDeviceToHost));
+  // the rows that failed up front: one check each, every evaluation exact
+  bqp_surrogate_stats& L = c->mlp->last;
+  L.net_evals = (int64_t)st[0];
+  L.exact_evals = (int64_t)tot[1] - (int64_t)st[0];
+  L.checks = (int64_t)st[2] + cnt[0];
+  L.check_failures = (int64_t)st[3] + cnt[0];
+  L.band_fallbacks = (int64_t)st[4];
+  if (stats) {
+    float ms = 0.0f;
+    HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    const double full = bqp_flops_per_rhs(c);
+    const double gfull = grav_flops(c->nmax, c->jmax);
+    const double low = grav_flops(std::min(c->mlp->nlow, c->nmax), c->mlp->low_jmax);
+    stats->steps = (int64_t)tot[0];
+    stats->evals = (int64_t)tot[1];
+    stats->rejects = (int64_t)tot[2];
+    stats->rows_ok = (int64_t)tot[3];
+    stats->rows_failed = (int64_t)tot[4];
+    stats->first_failed_row = (tot[4] > 0) ? first : -1;
+    stats->kernel_ms = ms;
+    stats->flops = (double)st[0] * (MLP_FLOPS + low + (full - gfull)) + (double)st[1] * (full + low) +
+                   ((double)tot[1] - (double)st[0] - (double)st[1]) * full;
+  }
+  if (tot[4] > 0) {
+    char buf[160];
+    snprintf(buf, sizeof(buf), "%llu row(s) failed; first failed row %d", tot[4], first);
+    c->err = buf;
+    return BQP_E_ROWS_FAILED;
+  }
+  return BQP_OK;
+}
+
 // Runs one propagation over device buffers; fills per-row outputs.
 int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
   if (c->nmax < 0 || !c->d_tab) return fail(c, BQP_E_NO_FIELD, "no gravity field loaded (call bqp_load_field)");
@@ -350,6 +474,9 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
     p.sur_check_every = 0;
     p.sur_totals = nullptr;
   }
+
+  if (sur && c->mlp->check_every > 0 && !c->launch.use_graph)
+    return run_surrogate_partitioned(c, p, n, kprec, G, kern, stats);
 
   const int64_t groups = (n + 63) / 64;
   int blocks = c->launch.blocks;
@@ -532,6 +659,7 @@ int32_t bqp_destroy(bqp_ctx* c) {
   (void)hipFree(c->d_first);
   (void)hipFree(c->d_stage);
   (void)hipFree(c->d_sort);
+  (void)hipFree(c->d_part);
   bqp_mlp_release(c);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->ev0) (void)hipEventDestroy(c->ev0);

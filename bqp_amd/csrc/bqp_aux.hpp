@@ -72,6 +72,88 @@ __global__ void __launch_bounds__(256) surrogate_data_kernel(FieldView full, Fie
   targ[3 * i + 2] = (float)((fz - lz) * is2);
 }
 
+// Surrogate pre-check: the first evaluation of every row is a sampled check
+// (evaluation 0 at the initial state), so it can be made for all rows before
+// the propagation.  fail[row] = 1 when the row's initial position is inside
+// the radius band and |low + network - exact| exceeds the threshold; such a
+// row would run the exact field for its whole arc.  One wave per block.
+template <int PREC>
+__global__ void __launch_bounds__(64) sur_precheck_kernel(const PropParams p, int n, int* __restrict__ fail) {
+  __shared__ uint4 sX[64 * 4];
+  __shared__ float sO[64 * 3];
+  const int lane = threadIdx.x;
+  const int row = blockIdx.x * 64 + lane;
+  const bool ok = row < n;
+  const int r = ok ? row : 0;
+  const double t = p.t0 ? p.t0[r] : p.t0c;
+  const double y0 = p.y0[(size_t)r * 6], y1 = p.y0[(size_t)r * 6 + 1], y2 = p.y0[(size_t)r * 6 + 2];
+  const double th = fma(p.model.omega, t, p.model.theta0);
+  double s, c;
+  sincos(th, &s, &c);
+  const double xb = fma(c, y0, s * y1);
+  const double yb = fma(c, y1, -(s * y0));
+  const double zb = y2;
+  double rx = 0.0, ry = 0.0, rz = 0.0;
+  sur_residual<PREC, 1>(p, 0, lane, sX, sO, xb, yb, zb, rx, ry, rz);
+  double fx, fy, fz, lx, ly, lz;
+  grav_fp64<1>(p.field, 0, xb, yb, zb, fx, fy, fz);
+  grav_fp64<1>(p.low, 0, xb, yb, zb, lx, ly, lz);
+  const double rad = sqrt(fma(xb, xb, fma(yb, yb, zb * zb)));
+  const bool band = (rad >= p.sur_r_lo) && (rad <= p.sur_r_hi);
+  const double dx = (lx + rx) - fx, dy = (ly + ry) - fy, dz = (lz + rz) - fz;
+  const bool bad = band && (fma(dx, dx, fma(dy, dy, dz * dz)) > p.sur_thresh2);
+  if (ok) fail[row] = bad ? 1 : 0;
+}
+
+// Stable split of the rows, taken in hand-out order (order[i], or i when
+// order is null), by fail[]: failing rows to exact_rows, the others to
+// sur_rows; counts[0..1] = sizes.  One block of 1024.
+__global__ void __launch_bounds__(1024) partition_kernel(const int* __restrict__ fail,
+                                                        const int32_t* __restrict__ order, int n,
+                                                        int* __restrict__ exact_rows, int* __restrict__ sur_rows,
+                                                        int* __restrict__ counts) {
+  __shared__ int scan[1024];
+  __shared__ int base_e, base_s;
+  if (threadIdx.x == 0) {
+    base_e = 0;
+    base_s = 0;
+  }
+  __syncthreads();
+  for (int off = 0; off < n; off += 1024) {
+    const int i = off + (int)threadIdx.x;
+    const int row = (i < n) ? (order ? order[i] : i) : 0;
+    const int f = (i < n) ? fail[row] : 0;
+    const int valid = (i < n) ? 1 : 0;
+    scan[threadIdx.x] = f;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {  // inclusive scan of the fail flags
+      const int v = (threadIdx.x >= (unsigned)d) ? scan[threadIdx.x - d] : 0;
+      __syncthreads();
+      scan[threadIdx.x] += v;
+      __syncthreads();
+    }
+    const int incl = scan[threadIdx.x];
+    const int chunk_fail = scan[1023];
+    if (valid) {
+      if (f)
+        exact_rows[base_e + incl - 1] = row;
+      else
+        sur_rows[base_s + (i - off) - incl] = row;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int m = min(1024, n - off);
+      base_e += chunk_fail;
+      base_s += m - chunk_fail;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    counts[0] = base_e;
+    counts[1] = base_s;
+  }
+}
+
 // Queue reset without hipMemsetAsync (graph-capture safe).
 __global__ void reset_counter_kernel(unsigned int* q) { q[0] = 0u; }
 

@@ -64,6 +64,8 @@ struct bqp_ctx {
   size_t stage_bytes = 0;
   void* d_sort = nullptr;
   size_t sort_bytes = 0;
+  void* d_part = nullptr;  // surrogate up-front check: flags and the two row lists
+  size_t part_bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   GraphKey* graph_key = nullptr;

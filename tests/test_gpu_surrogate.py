@@ -214,3 +214,37 @@ def test_network_beats_truncated_field(trained, field20):
     prop.reconfigure(precision="fp64")
     assert np.all(s4 == 0)
     assert np.abs(pos_err(y4, y)).max() < 1e-3 * np.median(e_net) + 1e-6
+
+
+def test_upfront_first_check_matches_in_kernel_checks(trained, field20):
+    """The first sampled check is made for all rows before the propagation
+    and failing rows run the plain FP64 kernel; with the same threshold the
+    in-kernel path (taken under HIP-graph replay) gives identical rows."""
+    announce("surrogate up-front check vs in-kernel checks: 64 rows x 1 h, threshold at the median error")
+    prop, sur, _, _, _ = trained
+    y0, t0, tf = _leo(64, 20, 3600.0)
+    pos = y0[:, :3]  # body frame = inertial at t = 0 (theta0 = 0)
+    full = prop.gravity_accel(pos)
+    with bqp_amd.Propagator(field20.truncated(4)) as p4:
+        low = p4.gravity_accel(pos)
+    sur.pack("bf16")
+    from bqp_amd.device import DeviceArray
+    f = DeviceArray.from_numpy(R.features(pos, R_LO, R_HI).astype(np.float32))
+    net, _ = sur.infer(f, "bf16")
+    err = np.linalg.norm(low + net.numpy() * sur.out_scale - full, axis=1)
+    thr = float(np.median(err))
+    sur.set_check(1_000_000_000, thr)  # only evaluation 0 is a check: the up-front split decides
+    prop.reconfigure(precision="surrogate")
+    prop.set_launch(waves_per_group=4, use_graph=0)
+    ya, sa, ia = prop.propagate_batch_final(y0, t0, tf)
+    st_a = sur.last_stats()
+    prop.set_launch(waves_per_group=4, use_graph=1)
+    yb, sb, ib = prop.propagate_batch_final(y0, t0, tf)
+    st_b = sur.last_stats()
+    prop.set_launch(waves_per_group=0, use_graph=0)
+    prop.reconfigure(precision="fp64")
+    assert np.all(sa == 0) and np.all(sb == 0)
+    assert 8 <= st_a["check_failures"] <= 56 and st_a["checks"] == 64
+    assert st_a["net_evals"] > 0
+    assert np.array_equal(ya, yb)
+    assert st_a == st_b and ia["evals"] == ib["evals"]
