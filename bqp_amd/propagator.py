@@ -124,9 +124,9 @@ class Propagator:
         """Launch shape: waves per trajectory group G (0 = auto), HIP-graph
         replay, persistent grid size (0 = fill the device), cost-sorted hand-out."""
         L = nat.Launch(waves_per_group, use_graph, blocks, sort_by_cost)
+        self._check(self._lib.bqp_set_launch(self._ctx, C.byref(L)))
         self.launch = {"waves_per_group": waves_per_group, "use_graph": use_graph, "blocks": blocks,
                        "sort_by_cost": sort_by_cost}
-        self._check(self._lib.bqp_set_launch(self._ctx, C.byref(L)))
 
     def reconfigure(self, **kw):
         """Change integrator/model settings (same keywords as the constructor)."""
