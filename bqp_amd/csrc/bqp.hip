@@ -165,8 +165,10 @@ void pack_field(int N, int M, const double* C, const double* S, double mu, doubl
 
 // Rescaled copy of an (a, b) table (the GravEntryL form): in each column s_n
 // is the running product of the recursion coefficients a_n (exact formulas in
-// long double; s = 1 on the column's first entry), g_n = b_n / (a_n a_{n-1})
-// and the summation coefficients are multiplied by s_n.
+// long double), started at the column's sectoral factor product
+// S_j = s_1 ... s_j (S_0 = 1) on its first entry; g_n = b_n / (a_n a_{n-1})
+// and the summation coefficients are multiplied by s_n.  The kernel then
+// starts column j from (x~ + i y~)^j R/r without the sectoral factors.
 void pack_lambda(int N, int jmax, const std::vector<GravEntry>& ab, std::vector<GravEntryL>& out) {
   out.clear();
   out.reserve(ab.size());
@@ -180,14 +182,16 @@ void pack_lambda(int N, int jmax, const std::vector<GravEntry>& ab, std::vector<
     return sqrtl((2 * dn + 1) * (dn + dm - 1) * (dn - dm - 1) / ((dn - dm) * (dn + dm) * (2 * dn - 3)));
   };
   size_t idx = 0;
+  long double sect = 1.0L;  // S_j = s_1 ... s_j: the column's sectoral factor product
   for (int j = 0; j <= jmax; ++j) {
     const int nlo = j == 0 ? 1 : j;
-    long double s = 1.0L, aprev = 0.0L;
+    if (j >= 1) sect *= (j == 1) ? sqrtl(3.0L) : sqrtl((2.0L * j + 1.0L) / (2.0L * j));
+    long double s = sect, aprev = 0.0L;
     for (int n = nlo; n <= N + 1; ++n, ++idx) {
       const GravEntry& e = ab[idx];
       GravEntryL L{};
-      long double scale = 1.0L;
-      if (!(j >= 1 && n == j)) {  // recursion entry (the sectoral entry keeps s = 1)
+      long double scale = sect;
+      if (!(j >= 1 && n == j)) {  // recursion entry (the sectoral entry keeps S_j)
         const long double a = a_nm(n, j), b = b_nm(n, j);
         s *= a;
         scale = s;

@@ -91,21 +91,11 @@ __device__ __forceinline__ int col_owner(int j) {
 }
 
 // Sectoral factors s_1 = sqrt(3), s_j = sqrt((2j+1)/(2j)) depend only on j;
-// they live in constant memory (filled at context creation) so the sectoral
-// chain every wave walks reads a few hundred bytes that stay in the scalar
-// cache, not the first entry of every column.
+// the split-precision form's sectoral chain reads them from constant memory
+// (filled at context creation).  The FP64 form has them folded into its
+// table (products S_j per column).
 constexpr int SECT_MAX = 208;
 static __constant__ double c_sect[SECT_MAX];
-
-// four consecutive sectoral factors, read with one scalar load
-struct Sect4 {
-  double v[4];
-};
-#if defined(__HIP_DEVICE_COMPILE__)
-typedef const __attribute__((address_space(4))) Sect4* CSect4;
-#else
-typedef const Sect4* CSect4;
-#endif
 static __constant__ float c_sectf[SECT_MAX];
 
 // One recursion step of a column in the rescaled form: U(n) from U(n-1),
@@ -123,6 +113,16 @@ static __constant__ float c_sectf[SECT_MAX];
     sz = fma((E).zv, VOUT, sz);                   \
     tz = fma((E).zw, WOUT, tz);                   \
   }
+// Contributions of a column's sectoral entry (V, W given).
+#define BQP_SECT_TERM(E, V, W) \
+  {                            \
+    sx = fma((E).xv, V, sx);   \
+    tx = fma((E).xw, W, tx);   \
+    sy = fma((E).yv, V, sy);   \
+    ty = fma((E).yw, W, ty);   \
+    sz = fma((E).zv, V, sz);   \
+    tz = fma((E).zw, W, tz);   \
+  }
 // Column 0 (W = 0): the V half only.
 #define BQP_COL0_STEP(E, VOUT, VPREV)    \
   {                                      \
@@ -132,6 +132,13 @@ static __constant__ float c_sectf[SECT_MAX];
     sy = fma((E).yv, VOUT, sy);          \
     sz = fma((E).zv, VOUT, sz);          \
   }
+
+// (ar + i ai) *= (br + i bi)
+__device__ __forceinline__ void cmul(double& ar, double& ai, double br, double bi) {
+  const double re = fma(ar, br, -(ai * bi));
+  ai = fma(ar, bi, ai * br);
+  ar = re;
+}
 
 // FP64 gravity, body-fixed position (m) -> acceleration (m/s^2), for the
 // columns owned by wave g of G.  The caller sums the G partial results.
@@ -143,14 +150,14 @@ __device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, d
   const double r2 = fma(x, x, fma(y, y, z * z));
   const double rr = 1.0 / r2;
   const double xt = x * R * rr, yt = y * R * rr, zt = z * R * rr, rho = R * R * rr;
-  double vs = R / sqrt(r2), ws = 0.0;
+  const double vs0 = R / sqrt(r2);
   // V-term and W-term sums are kept apart (two accumulation chains per
   // component, half the dependent-FMA latency per entry)
   double sx = 0.0, sy = 0.0, sz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
   const int top = f.nmax + 1;
   CEntryL e = (CEntryL)f.tabl;
-  if (col_owner<G>(0) == g) {
-    double v1 = vs, v2 = 0.0;
+  if (g == 0) {  // column 0 belongs to wave 0
+    double v1 = vs0, v2 = 0.0;
     CEntryL r = e;
     int k = top;
     for (; k >= 4; k -= 4, r += 4) {
@@ -170,55 +177,83 @@ __device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, d
       BQP_COL0_STEP(e0, v2, v1);
     }
   }
-  e += top;
-  // The sectoral factors are read four per scalar load (one wait per four
-  // columns instead of one per column): every wave walks the whole sectoral
-  // chain, whether it owns the columns or not.
-  for (int j0 = 1; j0 <= f.jmax; j0 += 4) {
-    const Sect4 sq = *(CSect4)(c_sect + j0);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      const int j = j0 + k;
-      if (j > f.jmax) break;
-      const double s = sq.v[k];
-      const double nv = s * fma(xt, vs, -(yt * ws));
-      const double nw = s * fma(xt, ws, yt * vs);
-      vs = nv;
-      ws = nw;
-      const int len = top + 1 - j;
-      if (col_owner<G>(j) == g) {
-        double v1 = vs, w1 = ws, v2 = 0.0, w2 = 0.0;
-        CEntryL r = e + 1;
-        int kk = len - 1;
-        {
-          // sectoral entry, loaded with the first batch of recursion entries
-          const GravEntryL es = e[0];
-          sx = fma(es.xv, vs, sx);
-          tx = fma(es.xw, ws, tx);
-          sy = fma(es.yv, vs, sy);
-          ty = fma(es.yw, ws, ty);
-          sz = fma(es.zv, vs, sz);
-          tz = fma(es.zw, ws, tz);
-        }
-        for (; kk >= 4; kk -= 4, r += 4) {
-          const GravEntryL e0 = r[0], e1 = r[1], e2 = r[2], e3 = r[3];
-          BQP_COL_STEP(e0, v2, v1, w2, w1);
-          BQP_COL_STEP(e1, v1, v2, w1, w2);
-          BQP_COL_STEP(e2, v2, v1, w2, w1);
-          BQP_COL_STEP(e3, v1, v2, w1, w2);
-        }
-        for (; kk >= 2; kk -= 2, r += 2) {
-          const GravEntryL e0 = r[0], e1 = r[1];
-          BQP_COL_STEP(e0, v2, v1, w2, w1);
-          BQP_COL_STEP(e1, v1, v2, w1, w2);
-        }
-        if (kk) {
-          const GravEntryL e0 = r[0];
-          BQP_COL_STEP(e0, v2, v1, w2, w1);
-        }
-      }
-      e += len;
+  // Columns j >= 1.  The sectoral value of column j is
+  //   V(j,j) + i W(j,j) = S_j (x~ + i y~)^j R/r,  S_j = s_1 s_2 ... s_j,
+  // and the host folds S_j into the column's coefficients, so a wave goes
+  // from one owned column to the next by one complex multiply with
+  // z^gap (z = x~ + i y~) instead of walking every column in between.
+  // Snake dealing gives wave g the columns g, 2G-1-g, 2G+g, 4G-1-g, ...:
+  // after the first, the gaps alternate ga = 2G-1-2g and gb = 2g+1.
+  const int ga = 2 * G - 1 - 2 * g, gb = 2 * g + 1;
+  double zgr = 1.0, zgi = 0.0;  // z^g
+  for (int k = 0; k < g; ++k) cmul(zgr, zgi, xt, yt);
+  double zbr = zgr, zbi = zgi;  // z^(2g+1)
+  cmul(zbr, zbi, zgr, zgi);
+  cmul(zbr, zbi, xt, yt);
+  double zar = xt, zai = yt;  // z^(2G-1-2g)
+  for (int k = 1; k < ga; ++k) cmul(zar, zai, xt, yt);
+  CEntryL e1 = e + top;  // column 1
+  int j = g;
+  bool next_a = g != 0;  // wave 0 starts from column 0 and takes gap ga first
+  double vs = vs0 * zgr, ws = vs0 * zgi;
+  if (g == 0) {
+    j = ga;
+    vs = vs0 * zar;
+    ws = vs0 * zai;
+    next_a = false;
+  }
+  while (j <= f.jmax) {
+    const int len = top + 1 - j;
+    const CEntryL ec = e1 + (col_offset(f.nmax, j) - top);
+    // The sectoral entry shares its scalar-load batch with the column's first
+    // three recursion entries (one wait for four entries; its terms are added
+    // after the three steps, so the compiler cannot hoist its load and wait
+    // above the branch); the three steps run with the registers swapped so
+    // the latest value ends in v1.
+    CEntryL r = ec + 1;
+    int kk = len - 1;
+    double v1, w1, v2, w2;
+    if (kk >= 3) {
+      const GravEntryL es = ec[0], e0 = r[0], e1q = r[1], e2 = r[2];
+      v2 = vs;
+      w2 = ws;
+      v1 = 0.0;
+      w1 = 0.0;
+      BQP_COL_STEP(e0, v1, v2, w1, w2);
+      BQP_COL_STEP(e1q, v2, v1, w2, w1);
+      BQP_COL_STEP(e2, v1, v2, w1, w2);
+      BQP_SECT_TERM(es, vs, ws);
+      kk -= 3;
+      r += 3;
+    } else {
+      const GravEntryL es = ec[0];
+      BQP_SECT_TERM(es, vs, ws);
+      v1 = vs;
+      w1 = ws;
+      v2 = 0.0;
+      w2 = 0.0;
     }
+    for (; kk >= 4; kk -= 4, r += 4) {
+      const GravEntryL e0 = r[0], e1q = r[1], e2 = r[2], e3 = r[3];
+      BQP_COL_STEP(e0, v2, v1, w2, w1);
+      BQP_COL_STEP(e1q, v1, v2, w1, w2);
+      BQP_COL_STEP(e2, v2, v1, w2, w1);
+      BQP_COL_STEP(e3, v1, v2, w1, w2);
+    }
+    for (; kk >= 2; kk -= 2, r += 2) {
+      const GravEntryL e0 = r[0], e1q = r[1];
+      BQP_COL_STEP(e0, v2, v1, w2, w1);
+      BQP_COL_STEP(e1q, v1, v2, w1, w2);
+    }
+    if (kk) {
+      const GravEntryL e0 = r[0];
+      BQP_COL_STEP(e0, v2, v1, w2, w1);
+    }
+    // next owned column (wave-uniform gap; the select is on a scalar flag)
+    const double pr = next_a ? zar : zbr, pi = next_a ? zai : zbi;
+    j += next_a ? ga : gb;
+    next_a = !next_a;
+    cmul(vs, ws, pr, pi);
   }
   ax = sx + tx;
   ay = sy + ty;

@@ -42,25 +42,33 @@ def test_w0_dopri5_vs_oracle(orc):
     assert np.mean(info["steps_per_row"] == steps) > 0.9
 
 
-def _oracle_ulp_spread(orc, field, w, n, method, rtol, drag, tf):
-    """The oracle against itself with x0 moved by one ulp: the conditioning
-    floor any two correct implementations can be expected to meet."""
+def _oracle_ulp_spread(orc, field, w, n, method, rtol, drag, tf, all_components=True):
+    """The oracle against itself with one component of the initial state moved
+    by one ulp, each of the six components up and down (12 runs; only x0 up
+    with all_components=False); per row the largest displacement: the
+    conditioning floor any two correct implementations can be expected to
+    meet.  One perturbation alone underestimates it on sensitive rows (a row
+    whose x0 + 1 ulp moves it by 3 mm moves by 3 cm under vx + 1 ulp)."""
     _, model, integ = oracle_pair(orc, field, method=method, rtol=rtol, drag=drag)
     bc = w["bc"][:n] if drag != "none" else None
     y0 = w["y0"][:n]
     a, _, _, _ = orc.propagate_batch(model, integ, y0, 0.0, tf, bc)
-    yp = y0.copy()
-    yp[:, 0] = np.nextafter(yp[:, 0], np.inf)
-    b, _, _, _ = orc.propagate_batch(model, integ, yp, 0.0, tf, bc)
-    return pos_err(a, b)
+    spread = np.zeros(n)
+    for comp in range(6 if all_components else 1):
+        for d in ((np.inf, -np.inf) if all_components else (np.inf,)):
+            yp = y0.copy()
+            yp[:, comp] = np.nextafter(yp[:, comp], d)
+            b, _, _, _ = orc.propagate_batch(model, integ, yp, 0.0, tf, bc)
+            spread = np.maximum(spread, pos_err(a, b))
+    return spread
 
 
 def test_rk87_drag_vs_oracle(orc):
     """With Vallado drag the piecewise scale heights make the final state
     sensitive at the centimetre level to ulp-sized differences (the oracle
-    against itself with a 1-ulp change of x0 spreads by ~1 cm over a day), so
-    the bar is: the median row agrees to 1e-5 m and the worst row stays within
-    3x the oracle's own 1-ulp spread (+1 mm)."""
+    against itself with one-ulp changes of the initial state spreads by up to
+    ~3 cm over a day), so the bar is: the median row agrees to 1e-5 m and the
+    worst row stays within 3x the oracle's own one-ulp spread (+1 mm)."""
     announce("RK8(7)13M N=20 Vallado drag 1 day, 64 rows vs oracle")
     w = inputs.workload("w1", n=64)
     field = bqp_amd.kaula(20)
@@ -113,7 +121,7 @@ def test_w1_seven_day_n70_drag_vs_oracle_spread(orc):
     yg, sg, info, yo, so, steps = _run_both(orc, field, w, 8, "rk87", 1e-12, "vallado")
     assert np.all(sg == 0) and np.all(so == 0)
     err = pos_err(yg, yo)
-    spread = _oracle_ulp_spread(orc, field, w, 8, "rk87", 1e-12, "vallado", 7 * DAY)
+    spread = _oracle_ulp_spread(orc, field, w, 8, "rk87", 1e-12, "vallado", 7 * DAY, all_components=False)
     announce(f"w1 drag parity max {err.max():.3e} median {np.median(err):.3e}; "
              f"oracle 1-ulp spread max {spread.max():.3e}")
     assert err.max() <= 3.0 * spread.max() + 1e-2
@@ -122,8 +130,11 @@ def test_w1_seven_day_n70_drag_vs_oracle_spread(orc):
 @pytest.mark.parametrize("G", [1, 2, 4])
 def test_launch_shape_bit_identity(G):
     """For a given number of waves per trajectory group, results do not depend
-    on grid size, hand-out order or queue order (bit-identical); different G
-    only reorder the gravity partial sums (1e-16-level differences)."""
+    on grid size, hand-out order or queue order (bit-identical).  Different G
+    reorder the gravity partial sums and reach the sectoral values by
+    different products (1e-16-level differences in the acceleration); where
+    such a difference flips a step-size decision the runs part by up to the
+    local error tolerance, so across G rows agree to 10 rtol |r|."""
     announce(f"launch shape invariance G={G}")
     w = inputs.workload("w2", n=700)
     field = bqp_amd.kaula(10)
@@ -137,7 +148,8 @@ def test_launch_shape_bit_identity(G):
         g1, _, _ = p.propagate_batch_final(w["y0"][:700], 0.0, tf)
     assert np.array_equal(s_ref, s_got)
     assert np.array_equal(ref, got)
-    assert pos_err(ref, g1).max() < 1e-4
+    r = np.linalg.norm(ref[:, :3], axis=1)
+    assert np.all(pos_err(ref, g1) <= 10 * 1e-11 * r)
 
 
 def test_batch_equals_single():
