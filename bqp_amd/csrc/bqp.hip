@@ -274,12 +274,14 @@ void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, prop_fn ker
 }
 
 // Algorithmic FLOPs of one FP64 gravity evaluation in the rescaled recursion
-// (FMA = 2): set-up 16, column 0 9 per entry (g rho, U, three sums), column
-// j >= 1 20 for the sectoral step and entry plus 19 per recursion entry
-// (g rho, U and W, six sums).
+// (FMA = 2, other operations 1): set-up 16, column 0 9 per entry (g rho, U,
+// three sums), column j >= 1 18 for its sectoral value (one complex
+// multiply) and the sectoral entry's six terms, plus 19 per recursion entry
+// (g rho, U and W, six sums).  The per-wave powers z^gap (at most 2G complex
+// multiplies per wave) are not counted.
 double grav_flops(int N, int jmax) {
   double f = 16.0 + 9.0 * (N + 1.0);
-  for (int j = 1; j <= jmax; ++j) f += 20.0 + 19.0 * (N + 1.0 - j);
+  for (int j = 1; j <= jmax; ++j) f += 18.0 + 19.0 * (N + 1.0 - j);
   return f;
 }
 
@@ -915,10 +917,6 @@ int32_t bqp_get_tableau(int32_t method, bqp_tableau* out) {
   return BQP_OK;
 }
 
-// Algorithmic FLOPs of one RHS evaluation (FMA = 2, mul/add/div/sqrt = 1),
-// counted from the column sweep above: setup 16, column 0 11 per entry,
-// columns j >= 1: 20 for the sectoral step + its contribution, 20 per
-// recursion entry; rotation 14; drag 30 when enabled.
 int32_t bqp_surrogate_dataset_dev(bqp_ctx* c, int64_t n, uint64_t seed, double* d_pos, float* d_feat,
                                   float* d_targ) {
   if (!c) return BQP_E_INVALID;
@@ -1000,6 +998,8 @@ int32_t bqp_surrogate_init(bqp_ctx* c, int32_t nlow, double r_lo, double r_hi, u
   return BQP_OK;
 }
 
+// Algorithmic FLOPs of one RHS evaluation: the gravity count (grav_flops)
+// plus rotation 14 and, when enabled, drag 30.
 double bqp_flops_per_rhs(const bqp_ctx* c) {
   if (!c || c->nmax < 0) return 0.0;
   double f = grav_flops(c->nmax, c->jmax);
