@@ -1,7 +1,7 @@
 """Launch-shape autotuner for the persistent propagation kernel.
 
 Tunables (``bqp_launch``): G, the number of waves that share one group of 64
-trajectories (1, 2, 4, 8; the workgroup is 64·G threads and holds 64
+trajectories (1, 2, 4, 8, 16; the workgroup is 64·G threads and holds 64
 trajectories), the persistent grid size, and the longest-first hand-out
 order.  The tuner runs a short pilot of the real batch — every row's span
 scaled by ``pilot_fraction`` so the spread of row lengths (and hence the
@@ -51,7 +51,7 @@ def autotune(prop, y0, t0, tf, bc=None, *, pilot_fraction: float = 0.01, min_pil
             prop.set_launch(**hit["launch"])
             return {"signature": key, "cached": True, **hit}
     if candidates is None:
-        candidates = [{"waves_per_group": g, "sort_by_cost": s} for g in (1, 2, 4, 8) for s in (1, 0)]
+        candidates = [{"waves_per_group": g, "sort_by_cost": s} for g in (1, 2, 4, 8, 16) for s in (1, 0)]
     pilot_tf = t0 + np.maximum(spans * pilot_fraction, np.minimum(spans, min_pilot_s))
     d_y0 = DeviceArray.from_numpy(y0)
     d_t0 = DeviceArray.from_numpy(np.ascontiguousarray(t0))

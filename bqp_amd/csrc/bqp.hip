@@ -238,7 +238,10 @@ int default_g(const bqp_ctx* c, int64_t n) {
   if (groups >= 4 * simds) return 1;
   if (groups >= 2 * simds) return 2;
   if (groups >= simds / 2) return 4;
-  return 8;
+  // Fewer groups than CUs: each group's workgroup has a CU to itself, and 16
+  // waves give it four per SIMD instead of two.
+  if (groups > (int64_t)c->cus) return 8;
+  return 16;
 }
 
 // Enqueues the whole call (order, queue reset, persistent kernel, row
@@ -306,7 +309,7 @@ int run_surrogate_partitioned(bqp_ctx* c, PropParams p, int64_t n, int kprec, in
                               bqp_stats* stats) {
   const bqp_config& cfg = c->cfg;
   prop_fn exact_kern = pick_kernel(cfg.method, 0, G);
-  if (!exact_kern) return fail(c, BQP_E_INVALID, "waves_per_group must be 1, 2, 4 or 8");
+  if (!exact_kern) return fail(c, BQP_E_INVALID, "waves_per_group must be 1, 2, 4, 8 or 16");
   int rc = ensure(c, &c->d_part, &c->part_bytes, (size_t)(3 * n + 64) * sizeof(int));
   if (rc) return rc;
   int* flags = (int*)c->d_part;
@@ -427,7 +430,7 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
   if (cfg.formulation == BQP_FORMULATION_ENCKE) kprec = 4;
   const int G = c->launch.waves_per_group > 0 ? c->launch.waves_per_group : default_g(c, n);
   prop_fn kern = pick_kernel(cfg.method, kprec, G);
-  if (!kern) return fail(c, BQP_E_INVALID, "waves_per_group must be 1, 2, 4 or 8");
+  if (!kern) return fail(c, BQP_E_INVALID, "waves_per_group must be 1, 2, 4, 8 or 16");
   if (cfg.precision == BQP_PRECISION_SPLIT && !c->d_tabf)
     return fail(c, BQP_E_NO_FIELD, "split precision needs the FP32 table (reload the field)");
 
@@ -687,8 +690,8 @@ int32_t bqp_set_config(bqp_ctx* c, const bqp_config* cfg) {
 int32_t bqp_set_launch(bqp_ctx* c, const bqp_launch* l) {
   if (!c || !l) return BQP_E_INVALID;
   if (l->waves_per_group != 0 && l->waves_per_group != 1 && l->waves_per_group != 2 && l->waves_per_group != 4 &&
-      l->waves_per_group != 8)
-    return fail(c, BQP_E_INVALID, "waves_per_group must be 0 (auto), 1, 2, 4 or 8");
+      l->waves_per_group != 8 && l->waves_per_group != 16)
+    return fail(c, BQP_E_INVALID, "waves_per_group must be 0 (auto), 1, 2, 4, 8 or 16");
   if (l->blocks < 0) return fail(c, BQP_E_INVALID, "blocks must be >= 0");
   if (l->use_graph != 0 && l->use_graph != 1) return fail(c, BQP_E_INVALID, "use_graph must be 0 or 1");
   c->launch = *l;

@@ -39,6 +39,7 @@ hipError_t upload_encke(const SecondOrderTableau* so, const double* sect, const 
       case 2: return propagate_kernel<M, P, 2>;                                                \
       case 4: return propagate_kernel<M, P, 4>;                                                \
       case 8: return propagate_kernel<M, P, 8>;                                                \
+      case 16: return propagate_kernel<M, P, 16>;                                              \
       default: return nullptr;                                                                 \
     }                                                                                          \
   }                                                                                            \

@@ -113,7 +113,7 @@ typedef struct bqp_stats {
 /* Launch shape (tuning); zero fields mean "default".  Chosen by the
  * autotuner in bqp_amd.autotune when not given. */
 typedef struct bqp_launch {
-  int32_t waves_per_group;  /* G: waves sharing one 64-trajectory group (0 = auto, 1, 2, 4, 8) */
+  int32_t waves_per_group;  /* G: waves sharing one 64-trajectory group (0 = auto, 1, 2, 4, 8, 16) */
   int32_t use_graph;        /* 1 = capture the call's launch sequence in a HIP graph once and
                                replay it while buffers, sizes and settings are unchanged */
   int32_t blocks;           /* persistent grid size; 0 = fill the device */

@@ -127,7 +127,7 @@ def test_w1_seven_day_n70_drag_vs_oracle_spread(orc):
     assert err.max() <= 3.0 * spread.max() + 1e-2
 
 
-@pytest.mark.parametrize("G", [1, 2, 4])
+@pytest.mark.parametrize("G", [1, 2, 4, 8, 16])
 def test_launch_shape_bit_identity(G):
     """For a given number of waves per trajectory group, results do not depend
     on grid size, hand-out order or queue order (bit-identical).  Different G

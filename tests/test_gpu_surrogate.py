@@ -149,7 +149,7 @@ def _leo(n, field_n, span):
     return w["y0"], np.zeros(n), np.full(n, span)
 
 
-@pytest.mark.parametrize("G", [1, 2, 4, 8])
+@pytest.mark.parametrize("G", [1, 2, 4, 8, 16])
 def test_failed_check_falls_back_bit_identically(field20, G):
     announce(f"surrogate fallback G={G}: threshold 0 -> every row exact; vs FP64 bitwise")
     y0, t0, tf = _leo(64, 20, 7200.0)
