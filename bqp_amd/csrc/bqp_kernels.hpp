@@ -725,6 +725,10 @@ propagate_kernel(const PropParams p) {
         }
       }
     }
+    // FSAL: on accept, wave 0 overwrites sA[0] with the last stage for the
+    // next step; every wave must have read sA[0] for the sums above first
+    // (a wave that lags behind wave 0 would otherwise read the new value)
+    if (MT::FSAL && G > 1) __syncthreads();
     bool finite = true;
 #pragma unroll
     for (int i = 0; i < 6; ++i) finite = finite && isfinite(yn[i]) && isfinite(err[i]);

@@ -152,6 +152,28 @@ def test_launch_shape_bit_identity(G):
     assert np.all(pos_err(ref, g1) <= 10 * 1e-11 * r)
 
 
+@pytest.mark.parametrize("G", [1, 2, 4, 8, 16])
+def test_fsal_every_g_matches_oracle(orc, G):
+    """DOPRI5 reuses the last stage as the next step's first (FSAL): wave 0
+    copies it into slot 0 of the group's LDS stage buffer only after every
+    wave has read slot 0 for its solution sums.  Regression test for a race
+    that let lagging waves read the copied value (four waves per SIMD at
+    G = 16): every row must finish and match the oracle at every G."""
+    announce(f"DOPRI5 compat FSAL, 16 rows x 3 h, N=8, G={G} vs oracle")
+    field = bqp_amd.kaula(8)
+    w = inputs.workload("w1", n=16, seed=8)
+    span = 3 * 3600.0
+    _, model, integ = oracle_pair(orc, field, method="dopri5", control="compat", eps=1e-9)
+    yo, so, steps, _ = orc.propagate_batch(model, integ, w["y0"], 0.0, span)
+    with bqp_amd.Propagator(field, method="dopri5", control="compat", eps=1e-9) as p:
+        p.set_launch(waves_per_group=G)
+        yg, sg, info = p.propagate_batch_final(w["y0"], 0.0, span)
+    assert np.all(sg == 0) and np.all(so == 0)
+    assert pos_err(yg, yo).max() < 1e-2
+    # same step sequences up to rare ulp-level accept/reject ties
+    assert np.max(np.abs(info["steps_per_row"] - steps) / steps) <= 0.01
+
+
 def test_batch_equals_single():
     announce("batch row == single-row call")
     w = inputs.workload("w0", n=130)
