@@ -506,6 +506,7 @@ def main():
             flops += (st["flops"] if form == "surrogate" else st["evals"] * frhs) + \
                 (st["steps"] + st["rejects"]) * fstage
         ok = int(np.sum(status.numpy() == 0))
+        ev_rows = evals_t.numpy()
         ms = wall / args.steps * 1e3
         kms = kern_ms / args.steps
         achieved = flops / args.steps / (kms * 1e-3) / 1e12
@@ -520,6 +521,10 @@ def main():
             "flops_per_rhs": frhs,
             "achieved_tflops": achieved,
             "launch": getattr(prop, "launch", None),
+            # per-row work spread: every lane holds one row, so the rows with
+            # the most evaluations set the launch's length
+            "evals_per_row": {"min": int(ev_rows.min()), "mean": float(ev_rows.mean()),
+                              "max": int(ev_rows.max())},
         }
         if form == "surrogate":
             results[form]["surrogate_stats"] = sur.last_stats()
@@ -603,6 +608,7 @@ def main():
             "kernel_ms": base["kernel_ms"],
             "flops_per_rhs": base["flops_per_rhs"],
             "rhs_evals": base["rhs_evals"],
+            "evals_per_row": base["evals_per_row"],
         },
         "cpu_baseline": cpu,
         "forms": forms_out,

@@ -734,13 +734,14 @@ int32_t bqp_load_field(bqp_ctx* c, int32_t nmax, int32_t mmax, const double* cba
   c->nmax = -1;
   c->h_cbar.assign(cbar, cbar + nt);
   c->h_sbar.assign(sbar, sbar + nt);
-  // two zero entries of padding: the unrolled column loops load entries in pairs
+  // zero entries of padding: the unrolled column loops load entries in
+  // pairs, and the pipelined FP64 loops read up to two entries past the
+  // last column
   tab.push_back(GravEntry{});
   tab.push_back(GravEntry{});
   tabf.push_back(GravEntryF{});
   tabf.push_back(GravEntryF{});
-  tabl.push_back(GravEntryL{});
-  tabl.push_back(GravEntryL{});
+  for (int k = 0; k < 4; ++k) tabl.push_back(GravEntryL{});
   HIP_TRY(c, hipMalloc(&c->d_tab, tab.size() * sizeof(GravEntry)));
   HIP_TRY(c, hipMalloc(&c->d_tabf, tabf.size() * sizeof(GravEntryF)));
   HIP_TRY(c, hipMemcpy(c->d_tab, tab.data(), tab.size() * sizeof(GravEntry), hipMemcpyHostToDevice));
@@ -965,8 +966,7 @@ int32_t bqp_surrogate_init(bqp_ctx* c, int32_t nlow, double r_lo, double r_hi, u
   int jmax = 0;
   pack_field(nlow, std::min(c->mmax, nlow), c->h_cbar.data(), c->h_sbar.data(), c->mu, c->radius, tab, jmax);
   pack_lambda(nlow, jmax, tab, tabl);
-  tabl.push_back(GravEntryL{});
-  tabl.push_back(GravEntryL{});
+  for (int k = 0; k < 4; ++k) tabl.push_back(GravEntryL{});  // padding, as for the field table
   (void)hipStreamSynchronize(c->stream);
   (void)hipFree(s->d_tab_low);
   s->d_tab_low = nullptr;
