@@ -31,10 +31,13 @@ def _run_both(orc, field, w, n, method, rtol, drag, tf=None, control="scaled", e
 
 
 def test_w0_dopri5_vs_oracle(orc):
-    announce("w0 DOPRI5 N=8 1 day, 256 rows vs oracle")
-    w = inputs.workload("w0", n=256)
+    """The CPU-runnable configuration in full: 1,024 seeded LEO orbits, one
+    day, DOPRI5(4) at rtol = atol = 1e-10 (canonical units), degree/order 8
+    Kaula field, no drag."""
+    announce("w0 DOPRI5 N=8 1 day, 1,024 rows (the whole workload) vs oracle")
+    w = inputs.workload("w0", n=1024)
     field = bqp_amd.kaula(8)
-    yg, sg, info, yo, so, steps = _run_both(orc, field, w, 256, "dopri5", 1e-10, "none")
+    yg, sg, info, yo, so, steps = _run_both(orc, field, w, 1024, "dopri5", 1e-10, "none")
     assert np.all(sg == 0) and np.all(so == 0)
     err = pos_err(yg, yo)
     assert err.max() < 1e-2, err.max()
