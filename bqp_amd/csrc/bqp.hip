@@ -214,6 +214,7 @@ FieldView field_view(const bqp_ctx* c) {
   f.tab = c->d_tab;
   f.tabf = c->d_tabf;
   f.tabl = c->d_tabl;
+  f.tablf = c->d_tablf;
   f.nmax = c->nmax;
   f.jmax = c->jmax;
   f.radius = c->radius;
@@ -468,6 +469,7 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
     p.low.tab = nullptr;
     p.low.tabl = s->d_tab_low;
     p.low.tabf = nullptr;
+    p.low.tablf = nullptr;
     p.low.nmax = std::min(s->nlow, c->nmax);
     p.low.jmax = s->low_jmax;
     p.mlp = s->packed;
@@ -663,6 +665,7 @@ int32_t bqp_destroy(bqp_ctx* c) {
   (void)hipFree(c->d_tab);
   (void)hipFree(c->d_tabf);
   (void)hipFree(c->d_tabl);
+  (void)hipFree(c->d_tablf);
   (void)hipFree(c->d_queue);
   (void)hipFree(c->d_totals);
   (void)hipFree(c->d_first);
@@ -728,9 +731,11 @@ int32_t bqp_load_field(bqp_ctx* c, int32_t nmax, int32_t mmax, const double* cba
   (void)hipFree(c->d_tab);
   (void)hipFree(c->d_tabf);
   (void)hipFree(c->d_tabl);
+  (void)hipFree(c->d_tablf);
   c->d_tab = nullptr;
   c->d_tabf = nullptr;
   c->d_tabl = nullptr;
+  c->d_tablf = nullptr;
   c->nmax = -1;
   c->h_cbar.assign(cbar, cbar + nt);
   c->h_sbar.assign(sbar, sbar + nt);
@@ -748,6 +753,17 @@ int32_t bqp_load_field(bqp_ctx* c, int32_t nmax, int32_t mmax, const double* cba
   HIP_TRY(c, hipMemcpy(c->d_tabf, tabf.data(), tabf.size() * sizeof(GravEntryF), hipMemcpyHostToDevice));
   HIP_TRY(c, hipMalloc(&c->d_tabl, tabl.size() * sizeof(GravEntryL)));
   HIP_TRY(c, hipMemcpy(c->d_tabl, tabl.data(), tabl.size() * sizeof(GravEntryL), hipMemcpyHostToDevice));
+  if (nmax >= PIPE_MIN_N && nmax <= SPLIT_RESCALED_MAX_N) {
+    // FP32 copy of the rescaled table for the split-precision form; its
+    // pipelined loops read up to four entries past the last column
+    std::vector<GravEntryLF> tablf(tabl.size() + 4, GravEntryLF{});
+    for (size_t i = 0; i < tabl.size(); ++i) {
+      const GravEntryL& e = tabl[i];
+      tablf[i] = GravEntryLF{(float)e.g, (float)e.xv, (float)e.xw, (float)e.yv, (float)e.yw, (float)e.zv, (float)e.zw};
+    }
+    HIP_TRY(c, hipMalloc(&c->d_tablf, tablf.size() * sizeof(GravEntryLF)));
+    HIP_TRY(c, hipMemcpy(c->d_tablf, tablf.data(), tablf.size() * sizeof(GravEntryLF), hipMemcpyHostToDevice));
+  }
   c->n_entries = tab.size();
   c->nmax = nmax;
   c->mmax = mmax;
@@ -933,6 +949,7 @@ int32_t bqp_surrogate_dataset_dev(bqp_ctx* c, int64_t n, uint64_t seed, double* 
   low.tab = nullptr;
   low.tabl = s->d_tab_low;
   low.tabf = nullptr;
+  low.tablf = nullptr;
   low.nmax = std::min(s->nlow, c->nmax);
   low.jmax = s->low_jmax;
   const double rmid = 0.5 * (s->r_lo + s->r_hi), rhalf = 0.5 * (s->r_hi - s->r_lo);

@@ -48,6 +48,13 @@ struct GravEntryL {
   double g, xv, xw, yv, yw, zv, zw;
 };
 
+// FP32 copy of the rescaled table (split precision, degree 30..90: the
+// running products s_n stay below ~1e28 there, so the rescaled values U =
+// V/s_n stay far inside the FP32 normal range).
+struct GravEntryLF {
+  float g, xv, xw, yv, yw, zv, zw;
+};
+
 // The table is read through the constant address space: loads from it are
 // invariant by definition, so uniform reads compile to scalar (SMEM) loads
 // even in kernels that also store to global memory.
@@ -56,10 +63,12 @@ struct GravEntryL {
 typedef const __attribute__((address_space(4))) GravEntry* CEntry;
 typedef const __attribute__((address_space(4))) GravEntryF* CEntryF;
 typedef const __attribute__((address_space(4))) GravEntryL* CEntryL;
+typedef const __attribute__((address_space(4))) GravEntryLF* CEntryLF;
 #else
 typedef const GravEntry* CEntry;
 typedef const GravEntryF* CEntryF;
 typedef const GravEntryL* CEntryL;
+typedef const GravEntryLF* CEntryLF;
 #endif
 
 // Offset of column j in the entry table of a degree-N field.  Column 0 holds
@@ -76,6 +85,7 @@ struct FieldView {
   const GravEntry* tab;    // FP64 entries, all columns ((a, b) form: split precision head)
   const GravEntryF* tabf;  // FP32 copy of the same entries (split precision)
   const GravEntryL* tabl;  // FP64 entries in the rescaled form (plain FP64 recursion)
+  const GravEntryLF* tablf;  // FP32 copy of tabl (split precision, degree 30..90), or null
   int nmax;                // field degree N
   int jmax;                // last column = min(N+1, M+1)
   double radius;
@@ -458,9 +468,12 @@ __device__ __forceinline__ void comp_add(Comp32& acc, float v) {
   acc.s = t;
 }
 
+// Split-precision gravity over the (a, b) table: walks every column's
+// sectoral chain (FP64 to SPLIT_N, FP32 after).  Used for fields of degree
+// below PIPE_MIN_N or above SPLIT_RESCALED_MAX_N.
 template <int G>
-__device__ __forceinline__ void grav_split(const FieldView& f, int g, double x, double y, double z,
-                                           double& ax, double& ay, double& az) {
+__device__ __forceinline__ void grav_split_ab(const FieldView& f, int g, double x, double y, double z,
+                                              double& ax, double& ay, double& az) {
   const double R = f.radius;
   const double r2 = fma(x, x, fma(y, y, z * z));
   const double rr = 1.0 / r2;
@@ -589,6 +602,241 @@ __device__ __forceinline__ void grav_split(const FieldView& f, int g, double x, 
   ax = sx + ((double)cx.s + (double)cx.c);
   ay = sy + ((double)cy.s + (double)cy.c);
   az = sz + ((double)cz.s + (double)cz.c);
+}
+
+// Largest degree for which the FP32 rescaled table is used (the running
+// products s_n of a column grow like 2^n: at degree 90 below 1e28, so U =
+// V/s_n stays above 1e-28, far from FP32's 1.2e-38 normal minimum).
+constexpr int SPLIT_RESCALED_MAX_N = 90;
+
+// FP32 recursion step of the rescaled table (as BQP_COL_STEP) into the FP32
+// column partial sums q (V terms) and u (W terms).
+#define BQP_COL_STEP_LF(E, VOUT, VPREV, WOUT, WPREV) \
+  {                                                  \
+    const float C_ = (E).g * rhof;                   \
+    VOUT = fmaf(ztf, VPREV, -(C_ * VOUT));           \
+    WOUT = fmaf(ztf, WPREV, -(C_ * WOUT));           \
+    qx = fmaf((E).xv, VOUT, qx);                     \
+    ux = fmaf((E).xw, WOUT, ux);                     \
+    qy = fmaf((E).yv, VOUT, qy);                     \
+    uy = fmaf((E).yw, WOUT, uy);                     \
+    qz = fmaf((E).zv, VOUT, qz);                     \
+    uz = fmaf((E).zw, WOUT, uz);                     \
+  }
+#define BQP_COL0_STEP_LF(E, VOUT, VPREV)   \
+  {                                        \
+    const float C_ = (E).g * rhof;         \
+    VOUT = fmaf(ztf, VPREV, -(C_ * VOUT)); \
+    qx = fmaf((E).xv, VOUT, qx);           \
+    qy = fmaf((E).yv, VOUT, qy);           \
+    qz = fmaf((E).zv, VOUT, qz);           \
+  }
+// One step of either form (column 0 has no W half).
+#define BQP_STEP_LF(E, VOUT, VPREV, WOUT, WPREV) \
+  {                                              \
+    if constexpr (COL0) {                        \
+      BQP_COL0_STEP_LF(E, VOUT, VPREV);          \
+    } else {                                     \
+      BQP_COL_STEP_LF(E, VOUT, VPREV, WOUT, WPREV); \
+    }                                            \
+  }
+
+// FP32 recursion entries r[0..cnt) of one column, four per batch with the
+// next batch in flight.  a0..a3 hold r[0..3] on entry (loaded by the caller
+// with the column's head, or past the column's end when cnt < 4: the table
+// is padded).  (v1, w1) is the latest value, (v2, w2) the one before.
+template <bool COL0>
+__device__ __forceinline__ void col_tail_lf(CEntryLF r, int cnt, GravEntryLF a0, GravEntryLF a1, GravEntryLF a2,
+                                            GravEntryLF a3, float ztf, float rhof, float& v1, float& v2, float& w1,
+                                            float& w2, float& qx, float& ux, float& qy, float& uy, float& qz,
+                                            float& uz) {
+  while (cnt >= 8) {
+    BQP_WAIT_SMEM();
+    BQP_SCHED_FENCE();
+    const GravEntryLF b0 = r[4], b1 = r[5], b2 = r[6], b3 = r[7];
+    BQP_SCHED_FENCE();
+    BQP_STEP_LF(a0, v2, v1, w2, w1);
+    BQP_STEP_LF(a1, v1, v2, w1, w2);
+    BQP_STEP_LF(a2, v2, v1, w2, w1);
+    BQP_STEP_LF(a3, v1, v2, w1, w2);
+    BQP_WAIT_SMEM();
+    BQP_SCHED_FENCE();
+    a0 = r[8];  // up to three entries past the column's end
+    a1 = r[9];
+    a2 = r[10];
+    a3 = r[11];
+    BQP_SCHED_FENCE();
+    BQP_STEP_LF(b0, v2, v1, w2, w1);
+    BQP_STEP_LF(b1, v1, v2, w1, w2);
+    BQP_STEP_LF(b2, v2, v1, w2, w1);
+    BQP_STEP_LF(b3, v1, v2, w1, w2);
+    r += 8;
+    cnt -= 8;
+  }
+  // cnt in 0..7; a0..a3 hold r[0..3]; the last 0..3 entries are issued
+  // before a0..a3 are computed
+  BQP_WAIT_SMEM();
+  BQP_SCHED_FENCE();
+  GravEntryLF b0, b1, b2;
+  if (cnt >= 5) b0 = r[4];
+  if (cnt >= 6) b1 = r[5];
+  if (cnt >= 7) b2 = r[6];
+  BQP_SCHED_FENCE();
+  if (cnt >= 1) BQP_STEP_LF(a0, v2, v1, w2, w1);
+  if (cnt >= 2) BQP_STEP_LF(a1, v1, v2, w1, w2);
+  if (cnt >= 3) BQP_STEP_LF(a2, v2, v1, w2, w1);
+  if (cnt >= 4) BQP_STEP_LF(a3, v1, v2, w1, w2);
+  BQP_WAIT_SMEM();
+  BQP_SCHED_FENCE();
+  if (cnt >= 5) BQP_STEP_LF(b0, v2, v1, w2, w1);
+  if (cnt >= 6) BQP_STEP_LF(b1, v1, v2, w1, w2);
+  if (cnt >= 7) BQP_STEP_LF(b2, v2, v1, w2, w1);
+}
+
+// Split-precision gravity over the rescaled tables (degree PIPE_MIN_N ..
+// SPLIT_RESCALED_MAX_N): the same column sweep, sectoral jumps and term
+// grouping as grav_fp64_pipelined.  Entries with n <= SPLIT_N (the central
+// term and J2..J4) run in FP64 from the FP64 table; the recursion continues
+// in FP32 from the FP32 copy (U values converted where a column crosses
+// SPLIT_N; sectoral values reached in FP64 and converted for columns past
+// SPLIT_N).  Each column's FP32 terms accumulate in two FP32 chains per
+// component, whose sum is folded into an FP32 Neumaier-compensated
+// accumulator; value and compensation are added to the FP64 sums at the end.
+template <int G>
+__device__ __forceinline__ void grav_split_pipelined(const FieldView& f, int g, double x, double y, double z,
+                                                     double& ax, double& ay, double& az) {
+  const double R = f.radius;
+  const double r2 = fma(x, x, fma(y, y, z * z));
+  const double rr = 1.0 / r2;
+  const double xt = x * R * rr, yt = y * R * rr, zt = z * R * rr, rho = R * R * rr;
+  const double vs0 = R / sqrt(r2);
+  const float ztf = (float)zt, rhof = (float)rho;
+  double sx = 0.0, sy = 0.0, sz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
+  Comp32 cx = {0.f, 0.f}, cy = {0.f, 0.f}, cz = {0.f, 0.f};
+  const int top = f.nmax + 1;
+  CEntryL e = (CEntryL)f.tabl;
+  CEntryLF ef = (CEntryLF)f.tablf;
+  if (g == 0) {  // column 0: n = k + 1 at entry k; FP64 for n <= SPLIT_N
+    double v1 = vs0, v2 = 0.0;
+    BQP_WAIT_SMEM();
+    BQP_SCHED_FENCE();
+    const GravEntryL h0 = e[0], h1 = e[1], h2 = e[2], h3 = e[3], h4 = e[4];
+    BQP_SCHED_FENCE();
+    BQP_COL0_STEP(h0, v2, v1);
+    BQP_COL0_STEP(h1, v1, v2);
+    BQP_COL0_STEP(h2, v2, v1);
+    BQP_COL0_STEP(h3, v1, v2);
+    BQP_COL0_STEP(h4, v2, v1);
+    float fv1 = (float)v2, fv2 = (float)v1, fw1 = 0.f, fw2 = 0.f;  // latest in v2
+    float qx = 0.f, ux = 0.f, qy = 0.f, uy = 0.f, qz = 0.f, uz = 0.f;
+    CEntryLF r = ef + SPLIT_N;
+    BQP_WAIT_SMEM();
+    BQP_SCHED_FENCE();
+    const GravEntryLF a0 = r[0], a1 = r[1], a2 = r[2], a3 = r[3];
+    BQP_SCHED_FENCE();
+    col_tail_lf<true>(r, top - SPLIT_N, a0, a1, a2, a3, ztf, rhof, fv1, fv2, fw1, fw2, qx, ux, qy, uy, qz, uz);
+    comp_add(cx, qx);
+    comp_add(cy, qy);
+    comp_add(cz, qz);
+  }
+  // columns j >= 1 by sectoral jumps (FP64), as in grav_fp64
+  const int ga = 2 * G - 1 - 2 * g, gb = 2 * g + 1;
+  double zgr = 1.0, zgi = 0.0;  // z^g
+  for (int k = 0; k < g; ++k) cmul(zgr, zgi, xt, yt);
+  double zbr = zgr, zbi = zgi;  // z^(2g+1)
+  cmul(zbr, zbi, zgr, zgi);
+  cmul(zbr, zbi, xt, yt);
+  double zar = xt, zai = yt;  // z^(2G-1-2g)
+  for (int k = 1; k < ga; ++k) cmul(zar, zai, xt, yt);
+  int j = g;
+  bool next_a = g != 0;
+  double vs = vs0 * zgr, ws = vs0 * zgi;
+  if (g == 0) {
+    j = ga;
+    vs = vs0 * zar;
+    ws = vs0 * zai;
+    next_a = false;
+  }
+  while (j <= f.jmax) {
+    const int off = col_offset(f.nmax, j);  // the column's sectoral entry
+    const int kk = top - j;                 // its recursion entries
+    float fv1, fv2, fw1, fw2;
+    float qx = 0.f, ux = 0.f, qy = 0.f, uy = 0.f, qz = 0.f, uz = 0.f;
+    if (j <= SPLIT_N) {
+      // FP64 sectoral term and recursion entries n = j+1 .. SPLIT_N
+      CEntryL ec = e + off;
+      const int h = SPLIT_N - j;
+      BQP_WAIT_SMEM();
+      BQP_SCHED_FENCE();
+      const GravEntryL es = ec[0];
+      BQP_SCHED_FENCE();
+      BQP_SECT_TERM(es, vs, ws);
+      double v1 = vs, w1 = ws, v2 = 0.0, w2 = 0.0;
+      for (int i = 0; i < h; ++i) {
+        const GravEntryL E = ec[1 + i];
+        BQP_COL_STEP(E, v2, v1, w2, w1);
+        const double tv = v1, tw = w1;  // latest back into v1 / w1
+        v1 = v2;
+        w1 = w2;
+        v2 = tv;
+        w2 = tw;
+      }
+      fv1 = (float)v1;
+      fw1 = (float)w1;
+      fv2 = (float)v2;
+      fw2 = (float)w2;
+      CEntryLF r = ef + off + 1 + h;
+      BQP_WAIT_SMEM();
+      BQP_SCHED_FENCE();
+      const GravEntryLF a0 = r[0], a1 = r[1], a2 = r[2], a3 = r[3];
+      BQP_SCHED_FENCE();
+      col_tail_lf<false>(r, kk - h, a0, a1, a2, a3, ztf, rhof, fv1, fv2, fw1, fw2, qx, ux, qy, uy, qz, uz);
+    } else {
+      // all FP32: the sectoral entry is loaded with the first four recursion entries
+      CEntryLF ec = ef + off;
+      BQP_WAIT_SMEM();
+      BQP_SCHED_FENCE();
+      const GravEntryLF es = ec[0];
+      const GravEntryLF a0 = ec[1], a1 = ec[2], a2 = ec[3], a3 = ec[4];
+      BQP_SCHED_FENCE();
+      BQP_WAIT_SMEM();
+      BQP_SCHED_FENCE();
+      const float vsf = (float)vs, wsf = (float)ws;
+      qx = es.xv * vsf;
+      ux = es.xw * wsf;
+      qy = es.yv * vsf;
+      uy = es.yw * wsf;
+      qz = es.zv * vsf;
+      uz = es.zw * wsf;
+      fv1 = vsf;
+      fw1 = wsf;
+      fv2 = 0.f;
+      fw2 = 0.f;
+      col_tail_lf<false>(ec + 1, kk, a0, a1, a2, a3, ztf, rhof, fv1, fv2, fw1, fw2, qx, ux, qy, uy, qz, uz);
+    }
+    comp_add(cx, qx + ux);
+    comp_add(cy, qy + uy);
+    comp_add(cz, qz + uz);
+    const double pr = next_a ? zar : zbr, pi = next_a ? zai : zbi;
+    j += next_a ? ga : gb;
+    next_a = !next_a;
+    cmul(vs, ws, pr, pi);
+  }
+  ax = (sx + tx) + ((double)cx.s + (double)cx.c);
+  ay = (sy + ty) + ((double)cy.s + (double)cy.c);
+  az = (sz + tz) + ((double)cz.s + (double)cz.c);
+}
+
+// Split-precision gravity: the rescaled pipelined form where its FP32 table
+// exists (degree PIPE_MIN_N .. SPLIT_RESCALED_MAX_N), else the (a, b) form.
+template <int G>
+__device__ __forceinline__ void grav_split(const FieldView& f, int g, double x, double y, double z, double& ax,
+                                           double& ay, double& az) {
+  if (f.tablf != nullptr) {
+    grav_split_pipelined<G>(f, g, x, y, z, ax, ay, az);
+    return;
+  }
+  grav_split_ab<G>(f, g, x, y, z, ax, ay, az);
 }
 
 // ------------------------------------------------------------- atmosphere

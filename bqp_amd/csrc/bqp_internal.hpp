@@ -53,6 +53,7 @@ struct bqp_ctx {
   double mu = 0.0, radius = 0.0;
   bqp::GravEntry* d_tab = nullptr;
   bqp::GravEntryL* d_tabl = nullptr;
+  bqp::GravEntryLF* d_tablf = nullptr;  // FP32 rescaled table (split precision, degree 30..90)
   bqp::GravEntryF* d_tabf = nullptr;
   size_t n_entries = 0;
   std::vector<double> h_cbar, h_sbar;  // host copy of the loaded coefficients
