@@ -241,6 +241,9 @@ __device__ __forceinline__ void sur_combine(const PropParams& p, SurLane& sl, bo
 // ENCKE_SEGMENT_S long (MAX_RECTIFICATION_SEGMENT_S, integrator.rs:236) and
 // each starts a fresh controller.
 constexpr double ENCKE_SEGMENT_S = 5400.0;
+// Step caps at or below this do not carry the step size across rebases
+// (HCARRY_UNCLAMPED_DT_MAX_S, integrator.rs:370).
+constexpr double HCARRY_MIN_CAP_S = 60.0;
 
 struct EnckeLane {
   double base[6];  // total state at the segment start (inertial, m, m/s)
@@ -503,8 +506,13 @@ propagate_kernel(const PropParams p) {
   };
 
   // Encke: start a segment at the current time from total state `tot`
-  // (deviation 0, fresh controller), integrator.rs:3418-3563.
-  auto start_segment = [&](const double* tot) {
+  // (deviation 0, fresh controller), integrator.rs:3418-3563.  `carry` > 0
+  // is the step size the controller asked for when the previous segment of
+  // the same row ended: the reference opens a rebased segment with it
+  // instead of the span/100 guess (hcarry_take / hcarry_store,
+  // integrator.rs:341-402), on its adaptive-solver path only (not the DOPRI5
+  // compat loop) and only when the step cap is above 60 s.
+  auto start_segment = [&](const double* tot, double carry) {
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
       el.base[i] = tot[i];
@@ -514,6 +522,7 @@ propagate_kernel(const PropParams p) {
     el.x = __builtin_nan("");
     el.seg_end = (tend - t > ENCKE_SEGMENT_S) ? t + ENCKE_SEGMENT_S : tend;
     start_controller(el.seg_end - t);
+    if (carry > 0.0 && !compat && !(p.h_max > 0.0 && p.h_max <= HCARRY_MIN_CAP_S)) h = clampd(carry, hmin, hmax);
     have_prev = just_rej = have_k1 = false;
     eprev = 0.0;
     rej_run = 0;
@@ -593,7 +602,7 @@ propagate_kernel(const PropParams p) {
               if (!two_body(p.mu, tot, 0.0, probe, x0)) {
                 finish(ROW_INVALID_INPUT);  // the baseline must be an ellipse
               } else {
-                start_segment(tot);
+                start_segment(tot, 0.0);  // a row's first segment has no predecessor
                 tseg = sampled ? fmin(p.t_eval[sidx], el.seg_end) : el.seg_end;
               }
             } else {
@@ -843,7 +852,7 @@ propagate_kernel(const PropParams p) {
           for (int i = 0; i < 6; ++i) y[i] = tot[i];
           finish(ROW_OK);
         } else {
-          if (t == el.seg_end) start_segment(tot);  // rebase
+          if (t == el.seg_end) start_segment(tot, h);  // rebase, carrying the controller's step
           tseg = sampled ? fmin(p.t_eval[sidx], el.seg_end) : el.seg_end;
           tcomp = 0.0;
         }

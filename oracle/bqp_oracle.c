@@ -532,10 +532,15 @@ static double radius_of(const double* y) { return sqrt(fma(y[0], y[0], fma(y[1],
  * LY/src/odesolve/solver.rs:1099-1925 integrate_internal with the PI
  * controller macros :445-529 (ORC_CONTROL_SCALED / _ABSOLUTE), or
  * LY/src/odesolve/lightyear_compat.rs:100-600 (ORC_CONTROL_COMPAT).  The step
- * budget is per call, as the reference's per-segment solver call. */
+ * budget is per call, as the reference's per-segment solver call.
+ * h_open > 0 replaces the solver's default opening step (clamped to
+ * [h_min, h_max]; the solver path's `h0`, LY/src/integrator.rs:490).  On
+ * return *h_exit holds the step size the controller asked for after its last
+ * accepted step (solver.rs:497-498 final_controller_h), 0 if none. */
 static int integrate_segment(rhs_ctx* rc, const orc_integ* cfg, const bqp_tableau* T, double y[6],
-                             double ta, double tb, orc_stats* loc) {
+                             double ta, double tb, orc_stats* loc, double h_open, double* h_exit) {
   const double span = tb - ta;
+  if (h_exit) *h_exit = 0.0;
   if (span == 0.0) return ORC_OK;
   int64_t seg_steps = 0, seg_rejects = 0;
   const int compat = (cfg->control == ORC_CONTROL_COMPAT);
@@ -551,7 +556,8 @@ static int integrate_segment(rhs_ctx* rc, const orc_integ* cfg, const bqp_tablea
     h_max = (cfg->h_max > 0.0 && isfinite(cfg->h_max)) ? cfg->h_max : span;
     h_min = (cfg->h_min > 0.0) ? cfg->h_min : 1e-12;
     if (h_max < h_min) h_max = h_min;
-    const double h0 = (cfg->h0 > 0.0) ? cfg->h0 : ((span <= 60.0) ? span / 2.0 : span / 100.0);
+    const double h0 = (h_open > 0.0) ? h_open
+                      : (cfg->h0 > 0.0) ? cfg->h0 : ((span <= 60.0) ? span / 2.0 : span / 100.0);
     h = clampd(h0, h_min, h_max);
   }
   const int64_t max_steps = cfg->max_steps > 0 ? cfg->max_steps : 500000;
@@ -684,6 +690,7 @@ static int integrate_segment(rhs_ctx* rc, const orc_integ* cfg, const bqp_tablea
       }
       fac = clampd(fac, 0.2, maxg);
       h = clampd(hs * fac, h_min, h_max);
+      if (h_exit) *h_exit = h;
       err_prev = en;
       have_prev = 1;
       just_rej = 0;
@@ -740,8 +747,17 @@ int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const do
   rc.md = md;
   rc.bc = bc;
   if (md->formulation != ORC_ENCKE) {
-    if (integrate_segment(&rc, cfg, T, y, t0, tf, &loc) != ORC_OK) goto fail;
+    if (integrate_segment(&rc, cfg, T, y, t0, tf, &loc, 0.0, NULL) != ORC_OK) goto fail;
   } else {
+    /* Step-size carry (LY/src/integrator.rs:341-402 hcarry_take/_store): a
+     * rebased segment opens with the step size the controller asked for at
+     * the end of the previous segment of the same row, on the adaptive
+     * solver path (not the DOPRI5 compat loop, :420-486) and only when the
+     * step cap is above 60 s (HCARRY_UNCLAMPED_DT_MAX_S, :370; h_max <= 0
+     * here means no cap).  The carry starts empty for every row. */
+    const int carry_ok = !cfg->no_hcarry && cfg->control != ORC_CONTROL_COMPAT &&
+                         !(cfg->h_max > 0.0 && cfg->h_max <= 60.0);
+    double carried = 0.0;
     double ts = t0;
     while (1) {
       const double te = (tf - ts > ORC_ENCKE_SEGMENT_S) ? ts + ORC_ENCKE_SEGMENT_S : tf;
@@ -754,7 +770,9 @@ int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const do
         loc.status = ORC_INVALID_INPUT;  /* baseline is not an ellipse */
         goto fail;
       }
-      if (integrate_segment(&rc, cfg, T, d, ts, te, &loc) != ORC_OK) goto fail;
+      double h_exit = 0.0;
+      if (integrate_segment(&rc, cfg, T, d, ts, te, &loc, carry_ok ? carried : 0.0, &h_exit) != ORC_OK) goto fail;
+      carried = h_exit;
       if (total_state(&rc, te, d, y) != ORC_OK) {
         loc.status = ORC_NON_FINITE;
         goto fail;

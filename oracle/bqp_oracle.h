@@ -98,6 +98,9 @@ typedef struct orc_integ {
   double h_max;     /* <= 0: span */
   int64_t max_steps;
   int32_t max_rejects;
+  int32_t no_hcarry;  /* 1: Encke segments always open with the default step
+                         (test switch; the reference carries it, see
+                         orc_propagate) */
 } orc_integ;
 
 typedef struct orc_stats {

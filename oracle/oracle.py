@@ -56,6 +56,7 @@ class Integ(C.Structure):
         ("h_max", C.c_double),
         ("max_steps", C.c_int64),
         ("max_rejects", C.c_int32),
+        ("no_hcarry", C.c_int32),
     ]
 
 
@@ -152,7 +153,7 @@ def make_model(field: Field, omega=7.292115e-5, theta0=0.0, drag="none", r_atm=6
 
 def make_integ(method="rk87", control="scaled", rtol=1e-12, atol_pos=None, atol_vel=None,
                eps=1e-8, h0=0.0, h_min=0.0, h_max=0.0, max_steps=500000, max_rejects=50,
-               canonical=(6378137.0, 3.986004418e14)) -> Integ:
+               canonical=(6378137.0, 3.986004418e14), hcarry=True) -> Integ:
     """Tolerances; atol defaults to rtol in canonical units (DU = R, TU = sqrt(R^3/mu))."""
     du, mu = canonical
     vu = np.sqrt(mu / du)
@@ -161,7 +162,7 @@ def make_integ(method="rk87", control="scaled", rtol=1e-12, atol_pos=None, atol_
     if atol_vel is None:
         atol_vel = rtol * vu
     return Integ(METHOD[method], CONTROL[control], rtol, atol_pos, atol_vel, eps, h0, h_min,
-                 h_max, max_steps, max_rejects)
+                 h_max, max_steps, max_rejects, 0 if hcarry else 1)
 
 
 def rhs(model: Model, bc, t, y):

@@ -33,9 +33,16 @@ def test_encke_vs_oracle_encke_one_day(orc, method):
     yo, so, steps = _oracle(orc, field, "encke", method, "absolute", 1e-7, 1e-12, "none", w["y0"], DAY)
     assert np.all(sg == 0) and np.all(so == 0)
     assert pos_err(yg, yo).max() < 1e-2
-    # the restated controller takes the same steps up to rare ulp-level
-    # accept/reject ties (16 controller restarts per day make ties likelier)
-    assert np.max(np.abs(info["steps_per_row"] - steps) / steps) <= 0.01
+    # The restated controller takes the same steps up to ulp-level
+    # accept/reject ties.  Each rebase opens with the step size carried over
+    # from the previous segment, so a tie's effect is not wiped out at the next
+    # rebase: the bar is the oracle's own step-count change when x0 moves by
+    # one ulp (measured 2.8% here; 0.4% with cold restarts).
+    yp = w["y0"].copy()
+    yp[:, 0] = np.nextafter(yp[:, 0], np.inf)
+    _, _, steps_ulp = _oracle(orc, field, "encke", method, "absolute", 1e-7, 1e-12, "none", yp, DAY)
+    ulp_spread = np.max(np.abs(steps_ulp - steps) / steps)
+    assert np.max(np.abs(info["steps_per_row"] - steps) / steps) <= max(0.01, 1.5 * ulp_spread)
 
 
 def test_encke_matches_cowell(orc):

@@ -108,3 +108,39 @@ def test_encke_detects_a_planted_battin_sign_error(orc):
     direct = field.mu * (base[:3] / np.linalg.norm(base[:3]) ** 3 - (base[:3] + dr) / np.linalg.norm(base[:3] + dr) ** 3)
     assert np.allclose(good, direct, rtol=1e-9)
     assert not np.allclose(-good, direct, rtol=1e-3)
+
+
+def test_step_size_carry_across_rebases(orc):
+    """A rebased segment opens with the step size the controller asked for at
+    the end of the previous one (LY/src/integrator.rs:341-402).  The carry
+    changes the step sequence (results differ from cold restarts, by less
+    than the Encke tolerance moves them from Cowell) at about the same cost
+    here (the span/100 opening guess is already near the controller's step),
+    and there is no carry where the reference has none: the DOPRI5 compat
+    loop and step caps of 60 s or less."""
+    field = bqp_amd.kaula(8)
+    w = inputs.workload("w1", n=8, seed=3)
+    f = orc.Field(field.cbar, field.sbar, field.nmax, field.mu, field.radius)
+    enc = orc.make_model(f, formulation="encke")
+    cow = orc.make_model(f)
+    can = (field.radius, field.mu)
+
+    def run(model, **kw):
+        integ = orc.make_integ(canonical=can, **kw)
+        y, s, steps, evals = orc.propagate_batch(model, integ, w["y0"], 0.0, 86400.0)
+        assert np.all(s == 0)
+        return y, evals
+
+    yc, _ = run(cow, method="rk87", rtol=1e-13)
+    y1, e1 = run(enc, method="rk87", control="absolute", eps=1e-7)
+    y0, e0 = run(enc, method="rk87", control="absolute", eps=1e-7, hcarry=False)
+    assert not np.array_equal(e1, e0) and not np.array_equal(y1, y0)
+    assert abs(e1.sum() / e0.sum() - 1.0) < 0.02, (e1, e0)
+    assert np.linalg.norm(y1[:, :3] - yc[:, :3], axis=1).max() < 1e-3
+    assert np.linalg.norm(y1[:, :3] - y0[:, :3], axis=1).max() < 1e-3
+    # no carry: a 60 s cap, or the compat loop -> identical to the switch turned off
+    for kw in ({"method": "rk87", "control": "absolute", "eps": 1e-7, "h_max": 60.0},
+               {"method": "dopri5", "control": "compat", "eps": 1e-9}):
+        ya, ea = run(enc, **kw)
+        yb, eb = run(enc, hcarry=False, **kw)
+        assert np.array_equal(ya, yb) and np.array_equal(ea, eb), kw
