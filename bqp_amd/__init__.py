@@ -17,7 +17,10 @@ path only): ``srp``/``sun``/``moon`` must be False; ``atm_model`` 1 is the
 reference's single-scale-height exponential model, 2 the Vallado piecewise
 table; ``method`` is one of dopri5 (reference compat controller), dop853,
 rk87 (Prince-Dormand RK8(7)13M).  The field is the GOCE DIR-R6 d/o-15 file the
-reference bakes in, truncated to ``gravity_order``.
+reference bakes in, truncated to ``gravity_order``.  As in the reference, the
+integrated state is the Encke deviation from a two-body baseline rebased
+every 5,400 s under absolute error control (``tol`` km); states in and out
+are inertial.
 """
 from __future__ import annotations
 
@@ -102,10 +105,13 @@ def _model(gravity_order, drag, srp, sun, moon, atm_model, am_ratio, cd, cr, tol
         raise ValueError(f"unknown integrator method {method!r}; use one of dopri5, dop853, rk87")
     meth, control = _METHOD_ALIASES[name]
     field = goce(int(gravity_order))
+    # the reference's equations of motion: Encke deviations from a two-body
+    # baseline rebased every 5,400 s, absolute error control on the deviation
+    # (DT/src/lib.rs:361-438 -> LY/src/integrator.rs:3418-3563)
     kw = dict(method=meth, control=control, drag=("none" if not drag else ("compat" if atm_model == 1 else
                                                                               "vallado")),
               eps=tol * 1e3, h_max=dt_max_s, theta0=gmst(epoch_jd), omega=OMEGA_EARTH, r_atm=R_EARTH,
-              r_min=r_min)
+              r_min=r_min, formulation="encke")
     return field, kw, am_ratio * cd
 
 
