@@ -58,6 +58,18 @@ def _six(value, what):
     return v
 
 
+def _bound_orbit(s, what):
+    """The reference converts the initial state to equinoctial elements first
+    and rejects states that are not a bound, non-degenerate ellipse
+    (DT/src/lib.rs:79-87 equinoctial_of)."""
+    r = float(np.linalg.norm(s[:3]))
+    v = float(np.linalg.norm(s[3:]))
+    h = float(np.linalg.norm(np.cross(s[:3], s[3:])))
+    energy = 0.5 * v * v - MU_KM / r if r > 0.0 else math.inf
+    if not (r > 0.0 and energy < 0.0 and h > 1e-12 * r * v):
+        raise ValueError(f"{what} is not a bound, non-degenerate elliptical orbit (equinoctial conversion failed)")
+
+
 def kep_to_eci(kep):
     """[a_km, e, i_deg, raan_deg, argp_deg, true_anomaly_deg] -> [x, y, z, vx, vy, vz] (km, km/s)."""
     k = _six(kep, "kep")
@@ -123,6 +135,7 @@ def propagate_final(state_eci, epoch_jd, tof_s, *, gravity_order=5, drag=True, s
     s = _six(state_eci, "state_eci")
     if not (math.isfinite(epoch_jd) and math.isfinite(tof_s) and tof_s > 0):
         raise ValueError("epoch_jd must be finite and tof_s > 0")
+    _bound_orbit(s, "state_eci")
     field, kw, bc = _model(gravity_order, drag, srp, sun, moon, atm_model, am_ratio, cd, cr, tol, dt_max_s,
                            method, epoch_jd, r_min=R_EARTH)
     with Propagator(field, **kw) as p:
@@ -133,6 +146,17 @@ def propagate_final(state_eci, epoch_jd, tof_s, *, gravity_order=5, drag=True, s
     if st != 0:
         raise RuntimeError(f"propagation failed: {ROW_STATUS.get(st, st)}")
     return yf[0] * 1e-3
+
+
+def _reached_samples(grid, states, status):
+    """Like the reference, an arc that stops early returns only the samples
+    it reached (DT/src/lib.rs:320-345: `completed` needs every sample).  A
+    failed row is NaN-filled from its first unreached sample on."""
+    if status == 0:
+        return np.array(grid), states
+    bad = ~np.all(np.isfinite(states), axis=1)
+    reached = int(np.argmax(bad)) if bad.any() else len(grid)
+    return np.array(grid[:reached]), states[:reached]
 
 
 def propagate(state_eci, epoch_jd, tof_s, times_s=None, *, gravity_order=5, drag=True, srp=False, sun=False,
@@ -154,6 +178,7 @@ def propagate(state_eci, epoch_jd, tof_s, times_s=None, *, gravity_order=5, drag
     if any(b <= a for a, b in zip(user, user[1:])) or any((not math.isfinite(t)) or t < 0 or t > tof_s
                                                          for t in user):
         raise ValueError("times_s must be strictly increasing, within [0, tof_s]")
+    _bound_orbit(s, "state_eci")
     grid = list(user)
     if not grid or grid[0] > 0.0:
         grid.insert(0, 0.0)
@@ -167,9 +192,10 @@ def propagate(state_eci, epoch_jd, tof_s, times_s=None, *, gravity_order=5, drag
                                                       bc if drag else None)
     wall = (time.perf_counter() - t0) * 1e6
     st = int(status[0])
+    times, states = _reached_samples(grid, out[0] * 1e-3, st)
     return {
-        "times_s": np.array(grid),
-        "states_eci": out[0] * 1e-3,
+        "times_s": times,
+        "states_eci": states,
         "completed": st == 0,
         "terminal_event": "" if st == 0 else ROW_STATUS.get(st, str(st)),
         "max_steps_exceeded": st == 1,

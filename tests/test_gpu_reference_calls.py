@@ -71,3 +71,4 @@ def test_propagate_batch_final_matches_oracle_and_flags_impact(orc):
     assert st[3] == 6 and np.all(np.isnan(yk[3]))
     with pytest.raises(ValueError, match="ground"):
         bqp_amd.propagate_final(s[3], EPOCH, tof[3])
+

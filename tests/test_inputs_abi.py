@@ -112,6 +112,15 @@ def test_python_boundary_validation_before_device():
         bqp_amd.propagate_final([1, 2, np.nan, 4, 5, 6], 2451545.0, 600.0)
     with pytest.raises(ValueError):
         bqp_amd.propagate(s, 2451545.0, 600.0, times_s=[100.0, 50.0])
+    # not a bound ellipse (above escape speed; radial, zero angular momentum)
+    fast = s.copy()
+    fast[3:] *= 1.5
+    radial = np.array([7000.0, 0.0, 0.0, 1.0, 0.0, 0.0])
+    for bad in (fast, radial):
+        with pytest.raises(ValueError, match="bound, non-degenerate"):
+            bqp_amd.propagate_final(bad, 2451545.0, 600.0)
+        with pytest.raises(ValueError, match="bound, non-degenerate"):
+            bqp_amd.propagate(bad, 2451545.0, 600.0)
 
 
 def _header_functions():
@@ -144,3 +153,17 @@ def test_host_only_entry_points_without_gpu():
     assert L.bqp_propagate_batch_final(None, None, None, None, None, 0, None, None, None, None, None) == \
         _native.E_INVALID
     assert L.bqp_last_error(None) == b"null context"
+
+
+def test_sampled_call_returns_only_the_samples_a_failed_arc_reached():
+    grid = [0.0, 600.0, 1200.0, 3600.0]
+    st = np.arange(24, dtype=float).reshape(4, 6)
+    t, y = bqp_amd._reached_samples(grid, st, 0)
+    assert list(t) == grid and np.array_equal(y, st)
+    failed = st.copy()
+    failed[2:] = np.nan  # the kernel NaN-fills from the first unreached sample
+    t, y = bqp_amd._reached_samples(grid, failed, 1)
+    assert list(t) == [0.0, 600.0] and np.array_equal(y, st[:2])
+    failed[:] = np.nan
+    t, y = bqp_amd._reached_samples(grid, failed, 5)
+    assert len(t) == 0 and y.shape == (0, 6)
