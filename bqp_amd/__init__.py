@@ -43,6 +43,8 @@ __all__ = [
 ]
 
 MU_KM = 398600.4415  # km^3/s^2, the reference's constant (satpy_core/src/lib.rs:74)
+WGS84_FLATTENING = 1.0 / 298.257223563
+GROUND_ALTITUDE_M = 100e3  # the reference's ground event altitude (lightyear_odeint_rs/src/types.rs:140)
 MAX_GRAVITY_ORDER = 15
 
 
@@ -99,7 +101,7 @@ _METHOD_ALIASES = {"dopri5": ("dopri5", "compat"), "dop853": ("dop853", "absolut
 
 
 def _model(gravity_order, drag, srp, sun, moon, atm_model, am_ratio, cd, cr, tol, dt_max_s, method, epoch_jd,
-           r_min):
+           ground):
     if not (isinstance(gravity_order, (int, np.integer)) and 1 <= gravity_order <= MAX_GRAVITY_ORDER):
         raise ValueError(f"gravity_order must be in 1..={MAX_GRAVITY_ORDER}, got {gravity_order}")
     if not (math.isfinite(am_ratio) and am_ratio > 0 and math.isfinite(cd) and cd > 0 and math.isfinite(cr)
@@ -123,7 +125,11 @@ def _model(gravity_order, drag, srp, sun, moon, atm_model, am_ratio, cd, cr, tol
     kw = dict(method=meth, control=control, drag=("none" if not drag else ("compat" if atm_model == 1 else
                                                                               "vallado")),
               eps=tol * 1e3, h_max=dt_max_s, theta0=gmst(epoch_jd), omega=OMEGA_EARTH, r_atm=R_EARTH,
-              r_min=r_min, formulation="encke")
+              formulation="encke")
+    if ground:
+        # the reference's terminal ground event: 100 km above the WGS84
+        # ellipsoid (lightyear_odeint_rs/src/events.rs:87-145)
+        kw.update(r_min=R_EARTH + GROUND_ALTITUDE_M, ground_flattening=WGS84_FLATTENING)
     return field, kw, am_ratio * cd
 
 
@@ -137,7 +143,7 @@ def propagate_final(state_eci, epoch_jd, tof_s, *, gravity_order=5, drag=True, s
         raise ValueError("epoch_jd must be finite and tof_s > 0")
     _bound_orbit(s, "state_eci")
     field, kw, bc = _model(gravity_order, drag, srp, sun, moon, atm_model, am_ratio, cd, cr, tol, dt_max_s,
-                           method, epoch_jd, r_min=R_EARTH)
+                           method, epoch_jd, ground=True)
     with Propagator(field, **kw) as p:
         yf, status, _ = p.propagate_batch_final(s[None, :] * 1e3, 0.0, float(tof_s), bc if drag else None)
     st = int(status[0])
@@ -146,6 +152,20 @@ def propagate_final(state_eci, epoch_jd, tof_s, *, gravity_order=5, drag=True, s
     if st != 0:
         raise RuntimeError(f"propagation failed: {ROW_STATUS.get(st, st)}")
     return yf[0] * 1e-3
+
+
+def _clears_min_radius(states_km, r_min_km):
+    """Radius and osculating perigee (MU_KM) of each row at least r_min_km
+    (a row whose perigee is not a positive finite number fails)."""
+    r = np.linalg.norm(states_km[:, :3], axis=1)
+    v = states_km[:, 3:]
+    h = np.cross(states_km[:, :3], v)
+    h2 = np.einsum("ij,ij->i", h, h)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        evec = np.cross(v, h) / MU_KM - states_km[:, :3] / r[:, None]
+        e = np.linalg.norm(evec, axis=1)
+        perigee = (h2 / MU_KM) / (1.0 + e)
+    return np.isfinite(r) & (r >= r_min_km) & np.isfinite(perigee) & (perigee > 0.0) & (perigee >= r_min_km)
 
 
 def _reached_samples(grid, states, status):
@@ -185,7 +205,7 @@ def propagate(state_eci, epoch_jd, tof_s, times_s=None, *, gravity_order=5, drag
     if grid[-1] < tof_s:
         grid.append(float(tof_s))
     field, kw, bc = _model(gravity_order, drag, srp, sun, moon, atm_model, am_ratio, cd, cr, tol, dt_max_s,
-                           method, epoch_jd, r_min=0.0)
+                           method, epoch_jd, ground=False)
     t0 = time.perf_counter()
     with Propagator(field, **kw) as p:
         out, status, info = p.propagate_batch_sampled(s[None, :] * 1e3, 0.0, np.array(grid),
@@ -215,7 +235,14 @@ def propagate_batch_final(states_eci, tof_s, *, epoch_jd=2451545.0, gravity_orde
     if not np.all(np.isfinite(tof)) or np.any(tof < 0):
         raise ValueError("tof_s must be finite and >= 0")
     field, kw, bc = _model(gravity_order, drag, srp, sun, moon, atm_model, am_ratio, cd, cr, tol, dt_max_s,
-                           method, epoch_jd, r_min=R_EARTH)
+                           method, epoch_jd, ground=True)
     with Propagator(field, **kw) as p:
         yf, status, _ = p.propagate_batch_final(y * 1e3, 0.0, tof, np.full(y.shape[0], bc) if drag else None)
+    # rows whose initial radius or osculating perigee is below R + 100 km are
+    # refused up front, as the reference's batch does (session.rs:1983-1995,
+    # 2140-2144): status below_min_radius, NaN-filled
+    low = ~_clears_min_radius(y, (R_EARTH + GROUND_ALTITUDE_M) * 1e-3)
+    status = status.copy()
+    status[low] = 6
+    yf[low] = np.nan
     return yf * 1e-3, status

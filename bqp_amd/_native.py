@@ -51,6 +51,7 @@ class Config(C.Structure):
         ("theta0", C.c_double),
         ("r_atm", C.c_double),
         ("r_min", C.c_double),
+        ("ground_flattening", C.c_double),
     ]
 
 

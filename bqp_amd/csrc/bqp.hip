@@ -92,6 +92,14 @@ bool valid_config(const bqp_config& c, std::string& why) {
     why = "omega, theta0, r_atm must be finite and r_min >= 0";
     return false;
   }
+  if (!(c.ground_flattening >= 0.0 && c.ground_flattening < 1.0)) {
+    why = "ground_flattening must be in [0, 1)";
+    return false;
+  }
+  if (c.ground_flattening > 0.0 && c.r_min > 0.0 && !(c.r_atm > 0.0)) {
+    why = "an ellipsoidal ground needs r_atm > 0 (its equatorial radius)";
+    return false;
+  }
   return true;
 }
 
@@ -452,6 +460,7 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
   p.model.theta0 = cfg.theta0;
   p.model.r_atm = cfg.r_atm;
   p.model.r_min = cfg.r_min;
+  p.model.flat = cfg.ground_flattening;
   p.model.drag = cfg.drag;
   p.control = cfg.control;
   p.max_steps = (int)std::min<int64_t>(cfg.max_steps > 0 ? cfg.max_steps : 500000, 1 << 29);
@@ -606,6 +615,7 @@ int32_t bqp_default_config(bqp_config* out) {
   out->theta0 = 0.0;
   out->r_atm = 6378137.0;
   out->r_min = 0.0;
+  out->ground_flattening = 0.0;
   return BQP_OK;
 }
 

@@ -884,6 +884,21 @@ __device__ __forceinline__ double density(int model, double h_m) {
 struct ModelView {
   double omega, theta0, r_atm, r_min;
   int drag;
+  double flat;  // ground ellipsoid flattening (0: the sphere r_min)
 };
+
+// The row's minimum-radius test at inertial position (x, y, z); r_min > 0.
+// flat == 0: |r| >= r_min.  flat > 0: the reference's ground event
+// (lightyear_odeint_rs/src/events.rs:87-145 check_ground): the height above
+// the ellipsoid of equatorial radius r_atm and polar radius r_atm (1 - flat),
+// measured along the position's direction, must be at least r_min - r_atm.
+__device__ __forceinline__ bool above_ground(const ModelView& m, double x, double y, double z) {
+  const double d = sqrt(fma(x, x, fma(y, y, z * z)));
+  if (m.flat == 0.0) return d >= m.r_min;
+  const double a = m.r_atm, b = m.r_atm * (1.0 - m.flat);
+  const double den = sqrt(fma(a * a, z * z, (b * b) * fma(x, x, y * y)));
+  const double surface = (den > 0.0) ? a * b * d / den : b;  // at the centre: the polar radius
+  return d - surface >= m.r_min - m.r_atm;
+}
 
 }  // namespace bqp

@@ -581,8 +581,7 @@ propagate_kernel(const PropParams p) {
           ok = ok && isfinite(t) && isfinite(tend) && (tend >= t);
           if (!ok) {
             finish(ROW_INVALID_INPUT);
-          } else if (p.model.r_min > 0.0 &&
-                     !(sqrt(fma(y[0], y[0], fma(y[1], y[1], y[2] * y[2]))) >= p.model.r_min)) {
+          } else if (p.model.r_min > 0.0 && !above_ground(p.model, y[0], y[1], y[2])) {
             finish(ROW_BELOW_MIN_RADIUS);
           } else {
             // samples that coincide with the start are written immediately
@@ -830,9 +829,7 @@ propagate_kernel(const PropParams p) {
       just_rej = false;
       double rmin_tot[6];
       if (p.model.r_min > 0.0) total_state(rmin_tot);
-      if (p.model.r_min > 0.0 &&
-          !(sqrt(fma(rmin_tot[0], rmin_tot[0], fma(rmin_tot[1], rmin_tot[1], rmin_tot[2] * rmin_tot[2]))) >=
-            p.model.r_min)) {
+      if (p.model.r_min > 0.0 && !above_ground(p.model, rmin_tot[0], rmin_tot[1], rmin_tot[2])) {
         finish(ROW_BELOW_MIN_RADIUS);
       } else if (ENCKE && t == tseg) {
         double tot[6];

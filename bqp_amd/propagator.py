@@ -51,12 +51,13 @@ class Propagator:
                  h0: float = 0.0, h_min: float = 0.0, h_max: float = 0.0, max_steps: int = 500000,
                  max_rejects: int = 50, omega: float = OMEGA_EARTH, theta0: float = 0.0,
                  r_atm: float = R_EARTH, r_min: float = 0.0, launch: dict | None = None,
-                 formulation: str = "cowell"):
+                 formulation: str = "cowell", ground_flattening: float = 0.0):
         self._lib = nat.load()
         self._ctx = C.c_void_p()
         self.field = field
         cfg = self._make_config(method, control, precision, drag, rtol, atol_pos, atol_vel, eps, h0, h_min,
-                                h_max, max_steps, max_rejects, omega, theta0, r_atm, r_min, field, formulation)
+                                h_max, max_steps, max_rejects, omega, theta0, r_atm, r_min, field, formulation,
+                                ground_flattening)
         rc = self._lib.bqp_create(C.byref(cfg), C.byref(self._ctx))
         if rc != 0:
             if rc == nat.E_INVALID:
@@ -71,7 +72,8 @@ class Propagator:
     # ------------------------------------------------------------ plumbing
     @staticmethod
     def _make_config(method, control, precision, drag, rtol, atol_pos, atol_vel, eps, h0, h_min, h_max,
-                     max_steps, max_rejects, omega, theta0, r_atm, r_min, field, formulation="cowell") -> nat.Config:
+                     max_steps, max_rejects, omega, theta0, r_atm, r_min, field, formulation="cowell",
+                     ground_flattening=0.0) -> nat.Config:
         for name, table, val in (("method", nat.METHODS, method), ("control", nat.CONTROLS, control),
                                  ("precision", nat.PRECISIONS, precision), ("drag", nat.DRAGS, drag),
                                  ("formulation", nat.FORMULATIONS, formulation)):
@@ -93,6 +95,7 @@ class Propagator:
         cfg.h0, cfg.h_min, cfg.h_max = h0, h_min, h_max
         cfg.max_steps, cfg.max_rejects = int(max_steps), int(max_rejects)
         cfg.omega, cfg.theta0, cfg.r_atm, cfg.r_min = omega, theta0, r_atm, r_min
+        cfg.ground_flattening = ground_flattening
         return cfg
 
     def _check(self, rc: int, allow_rows: bool = False):

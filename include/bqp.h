@@ -97,6 +97,11 @@ typedef struct bqp_config {
   double theta0;       /* body-frame angle at t = 0 (rad) */
   double r_atm;        /* altitude = |r| - r_atm (m) */
   double r_min;        /* rows whose |r| drops below r_min fail (0 = off) */
+  double ground_flattening; /* 0: the r_min test is the sphere |r| >= r_min.  f > 0: it is
+                               the reference's ground event (lightyear_odeint_rs/src/
+                               events.rs:87-145): altitude above the ellipsoid of
+                               equatorial radius r_atm and flattening f, along the
+                               position's direction, at least r_min - r_atm */
 } bqp_config;
 
 typedef struct bqp_stats {

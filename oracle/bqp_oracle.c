@@ -528,6 +528,21 @@ static void rk_step(rhs_ctx* rc, const bqp_tableau* T, double t,
 
 static double radius_of(const double* y) { return sqrt(fma(y[0], y[0], fma(y[1], y[1], y[2] * y[2]))); }
 
+/* The minimum-radius test (r_min > 0).  With a flattening f > 0 it is the
+ * reference's ground event, LY/src/events.rs:87-145 check_ground: the surface
+ * radius of the ellipsoid (equatorial r_atm, polar r_atm (1 - f)) along the
+ * position's direction is R = a b |r| / sqrt(a^2 z^2 + b^2 (x^2 + y^2)), and
+ * the row fails once |r| - R drops below r_min - r_atm (the reference's
+ * GROUND_ALTITUDE, 100 km). */
+static int above_ground(const orc_model* md, const double* y) {
+  const double d = radius_of(y);
+  if (md->ground_flattening == 0.0) return d >= md->r_min;
+  const double a = md->r_atm, b = md->r_atm * (1.0 - md->ground_flattening);
+  const double den = sqrt(fma(a * a, y[2] * y[2], (b * b) * fma(y[0], y[0], y[1] * y[1])));
+  const double surface = (den > 0.0) ? a * b * d / den : b;
+  return d - surface >= md->r_min - md->r_atm;
+}
+
 /* One adaptive integration of y over [ta, tb] with a fresh controller:
  * LY/src/odesolve/solver.rs:1099-1925 integrate_internal with the PI
  * controller macros :445-529 (ORC_CONTROL_SCALED / _ABSOLUTE), or
@@ -664,7 +679,7 @@ static int integrate_segment(rhs_ctx* rc, const orc_integ* cfg, const bqp_tablea
       memcpy(y, yn, 6 * sizeof(double));
       if (rc->md->r_min > 0.0) {
         double tot[6];
-        if (total_state(rc, t, y, tot) != ORC_OK || !(radius_of(tot) >= rc->md->r_min)) {
+        if (total_state(rc, t, y, tot) != ORC_OK || !above_ground(rc->md, tot)) {
           loc->status = ORC_BELOW_MIN_RADIUS;
           return loc->status;
         }
@@ -738,7 +753,7 @@ int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const do
     loc.status = ORC_INVALID_INPUT;
     goto fail;
   }
-  if (md->r_min > 0.0 && !(radius_of(y) >= md->r_min)) {
+  if (md->r_min > 0.0 && !above_ground(md, y)) {
     loc.status = ORC_BELOW_MIN_RADIUS;
     goto fail;
   }

@@ -80,6 +80,9 @@ typedef struct orc_model {
   double r_atm;     /* radius subtracted from |r| to get altitude, m */
   double r_min;     /* minimum admissible radius, m (0 disables) */
   int formulation;  /* ORC_COWELL or ORC_ENCKE */
+  double ground_flattening; /* 0: |r| >= r_min; f > 0: height above the
+                               ellipsoid (equatorial radius r_atm, flattening
+                               f) along r at least r_min - r_atm */
 } orc_model;
 
 /* Equations of motion: Cowell (inertial state) or Encke (deviation from a

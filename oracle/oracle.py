@@ -40,6 +40,7 @@ class Model(C.Structure):
         ("r_atm", C.c_double),
         ("r_min", C.c_double),
         ("formulation", C.c_int),
+        ("ground_flattening", C.c_double),
     ]
 
 
@@ -145,8 +146,8 @@ FORMULATION = {"cowell": 0, "encke": 1}
 
 
 def make_model(field: Field, omega=7.292115e-5, theta0=0.0, drag="none", r_atm=6378137.0,
-               r_min=0.0, formulation="cowell") -> Model:
-    m = Model(field.ptr, omega, theta0, DRAG[drag], r_atm, r_min, FORMULATION[formulation])
+               r_min=0.0, formulation="cowell", ground_flattening=0.0) -> Model:
+    m = Model(field.ptr, omega, theta0, DRAG[drag], r_atm, r_min, FORMULATION[formulation], ground_flattening)
     m._field_ref = field  # the C field must outlive every model that points at it
     return m
 

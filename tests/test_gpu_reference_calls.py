@@ -18,11 +18,12 @@ pytestmark = pytest.mark.gpu
 EPOCH = 2459000.5
 
 
-def _oracle(orc, s_km, span, gravity_order=5, tol=1e-8, am_ratio=0.01, cd=2.2, r_min=R_EARTH):
+def _oracle(orc, s_km, span, gravity_order=5, tol=1e-8, am_ratio=0.01, cd=2.2, ground=True):
     field = bqp_amd.goce(gravity_order)
     f = orc.Field(field.cbar, field.sbar, field.nmax, field.mu, field.radius)
     m = orc.make_model(f, omega=OMEGA_EARTH, theta0=bqp_amd.gmst(EPOCH), drag="compat", r_atm=R_EARTH,
-                       r_min=r_min, formulation="encke")
+                       r_min=(R_EARTH + 100e3) if ground else 0.0, formulation="encke",
+                       ground_flattening=bqp_amd.WGS84_FLATTENING if ground else 0.0)
     i = orc.make_integ(method="dop853", control="absolute", eps=tol * 1e3, h_max=300.0)
     y0 = np.asarray(s_km, dtype=np.float64).reshape(-1, 6) * 1e3
     yo, so, _, _ = orc.propagate_batch(m, i, y0, 0.0, span, np.full(y0.shape[0], am_ratio * cd))
@@ -49,7 +50,7 @@ def test_propagate_samples_match_final_and_oracle(orc):
     assert list(out["times_s"]) == [0.0, 600.0, 5400.0, 9000.0, tof]
     assert np.array_equal(out["states_eci"][0], s)
     assert out["steps"] > 0 and out["rhs_evals"] >= out["steps"]
-    yo, so = _oracle(orc, s, tof, r_min=0.0)
+    yo, so = _oracle(orc, s, tof, ground=False)
     assert so[0] == 0
     assert np.linalg.norm(out["states_eci"][-1, :3] - yo[0, :3]) * 1e3 < 1e-2
     yf = bqp_amd.propagate_final(s, EPOCH, tof)
@@ -67,8 +68,25 @@ def test_propagate_batch_final_matches_oracle_and_flags_impact(orc):
         yo_k, so_k = _oracle(orc, s[k:k + 1], tof[k])
         assert st[k] == 0 and so_k[0] == 0
         assert np.linalg.norm(yk[k, :3] - yo_k[0, :3]) * 1e3 < 1e-2
-    # perigee 6500 km x (1 - 0.02) is below the Earth's surface: flagged, NaN-filled
+    # perigee 6500 km x (1 - 0.02) is below R + 100 km: refused up front,
+    # NaN-filled; the single-arc call reaches the ground event in flight
     assert st[3] == 6 and np.all(np.isnan(yk[3]))
     with pytest.raises(ValueError, match="ground"):
         bqp_amd.propagate_final(s[3], EPOCH, tof[3])
+
+
+def test_ground_event_over_the_ellipsoid_matches_oracle(orc):
+    announce("ground event 100 km above WGS84: polar state at 110 km clears it, the sphere would not")
+    R, fl = R_EARTH, bqp_amd.WGS84_FLATTENING
+    field = bqp_amd.kaula(4)
+    rp = R * (1.0 - fl) + 110e3
+    polar = np.array([[0.0, 0.0, rp, np.sqrt(field.mu / rp), 0.0, 0.0]])
+    for flat, want in ((fl, 0), (0.0, 6)):
+        with bqp_amd.Propagator(field, r_min=R + 100e3, ground_flattening=flat) as p:
+            _, st, _ = p.propagate_batch_final(polar, 0.0, 60.0)
+        f = orc.Field(field.cbar, field.sbar, field.nmax, field.mu, field.radius)
+        m = orc.make_model(f, r_atm=R, r_min=R + 100e3, ground_flattening=flat)
+        i = orc.make_integ(method="rk87", rtol=1e-12, canonical=(field.radius, field.mu))
+        _, so, _, _ = orc.propagate_batch(m, i, polar, 0.0, 60.0)
+        assert int(st[0]) == want and int(so[0]) == want
 

@@ -167,3 +167,14 @@ def test_sampled_call_returns_only_the_samples_a_failed_arc_reached():
     failed[:] = np.nan
     t, y = bqp_amd._reached_samples(grid, failed, 5)
     assert len(t) == 0 and y.shape == (0, 6)
+
+
+def test_batch_min_radius_precheck():
+    """Rows are refused up front when the radius or the osculating perigee is
+    below R + 100 km (the reference's batch, session.rs:1983-1995)."""
+    r_min = 6378.137 + 100.0
+    good = bqp_amd.kep_to_eci([6878.0, 0.001, 51.6, 0.0, 0.0, 0.0])
+    low_perigee = bqp_amd.kep_to_eci([6600.0, 0.03, 51.6, 0.0, 0.0, 180.0])  # at apogee 6798 km, perigee 6402 km
+    low_now = bqp_amd.kep_to_eci([6450.0, 0.0, 51.6, 0.0, 0.0, 0.0])
+    ok = bqp_amd._clears_min_radius(np.array([good, low_perigee, low_now]), r_min)
+    assert list(ok) == [True, False, False]

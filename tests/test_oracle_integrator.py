@@ -173,3 +173,36 @@ def test_solve_kepler_identity():
     for e in (0.0, 0.1, 0.7, 0.95):
         E = solve_kepler(M, np.full_like(M, e))
         assert np.max(np.abs(E - e * np.sin(E) - M)) < 1e-14
+
+
+def test_ground_event_over_the_wgs84_ellipsoid(orc):
+    """With a flattening the minimum-radius test is the reference's ground
+    event (LY/src/events.rs:87-145): height above the ellipsoid along the
+    position's direction.  Over a pole the WGS84 surface is 21.4 km inside
+    the equatorial sphere, so a state 110 km above the polar surface clears
+    the ellipsoid's 100 km floor but not the sphere R + 100 km; over the
+    equator both agree."""
+    import bqp_amd
+
+    field = bqp_amd.kaula(4)
+    f = orc.Field(field.cbar, field.sbar, field.nmax, field.mu, field.radius)
+    R, fl = 6378137.0, 1.0 / 298.257223563
+    b = R * (1.0 - fl)
+    integ = orc.make_integ(method="rk87", rtol=1e-10, canonical=(field.radius, field.mu))
+
+    def status(y0, flat):
+        m = orc.make_model(f, r_atm=R, r_min=R + 100e3, ground_flattening=flat)
+        _, st, _, _ = orc.propagate_batch(m, integ, np.asarray(y0, dtype=float)[None, :], 0.0, 60.0)
+        return int(st[0])
+
+    rp = b + 110e3
+    vp = np.sqrt(field.mu / rp)
+    polar = [0.0, 0.0, rp, vp, 0.0, 0.0]
+    assert status(polar, fl) == 0
+    assert status(polar, 0.0) == 6
+    re = R + 90e3
+    equatorial = [re, 0.0, 0.0, 0.0, np.sqrt(field.mu / re), 0.0]
+    assert status(equatorial, fl) == 6 and status(equatorial, 0.0) == 6
+    re = R + 110e3
+    equatorial = [re, 0.0, 0.0, 0.0, np.sqrt(field.mu / re), 0.0]
+    assert status(equatorial, fl) == 0 and status(equatorial, 0.0) == 0
