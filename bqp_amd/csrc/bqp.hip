@@ -242,10 +242,12 @@ int default_g(const bqp_ctx* c, int64_t n) {
   // Split each 64-trajectory group over enough waves that about four waves
   // are resident per SIMD (the occupancy the kernel is compiled for); measured
   // at 65,536 rows x degree 70: G=1 8.8, G=2 19.9, G=4 29.3, G=8 28.6 TFLOP/s.
+  // Larger batches keep G = 4 (the persistent grid then refills lanes): at
+  // 262,144 rows x 6 h x degree 70, G = 1 / 2 / 4 / 8 measured 47.7k / 49.6k /
+  // 55.3k / 51.6k orbit-days/s (the G = 1 kernel holds only 2 waves per SIMD:
+  // it needs 232 VGPRs).
   const int64_t groups = (n + 63) / 64;
   const int64_t simds = (int64_t)c->cus * 4;
-  if (groups >= 4 * simds) return 1;
-  if (groups >= 2 * simds) return 2;
   if (groups >= simds / 2) return 4;
   // Fewer groups than CUs: each group's workgroup has a CU to itself, and 16
   // waves give it four per SIMD instead of two.
