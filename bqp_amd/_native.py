@@ -23,6 +23,10 @@ ROW_STATUS = {
     5: "invalid_input",
     6: "below_min_radius",
 }
+# cells of the learned cost model (include/bqp.h BQP_COST_ALT_BINS, BQP_COST_INC_BINS)
+COST_ALT_BINS = 96
+COST_INC_BINS = 4
+
 METHODS = {"dopri5": 0, "dop853": 1, "rk87": 2, "dopri87": 2, "rk8(7)13m": 2}
 CONTROLS = {"scaled": 0, "absolute": 1, "compat": 2}
 DRAGS = {"none": 0, "vallado": 1, "compat": 2}
@@ -109,6 +113,7 @@ EXPORTS = [
     "bqp_propagate_batch_sampled",
     "bqp_get_tableau",
     "bqp_flops_per_rhs",
+    "bqp_get_cost_model",
     "bqp_surrogate_param_count",
     "bqp_surrogate_init",
     "bqp_surrogate_get_params",
@@ -161,6 +166,7 @@ def load(path: str | None = None):
     L.bqp_get_tableau.argtypes = [C.c_int32, C.POINTER(Tableau)]
     L.bqp_flops_per_rhs.restype = C.c_double
     L.bqp_flops_per_rhs.argtypes = [C.c_void_p]
+    L.bqp_get_cost_model.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     _fp = C.POINTER(C.c_float)
     vp = C.c_void_p
     L.bqp_surrogate_param_count.restype = C.c_int64

@@ -255,6 +255,26 @@ int default_g(const bqp_ctx* c, int64_t n) {
   return 16;
 }
 
+// The learned cost model when the hand-out order uses it, else nullptr.
+CostTable* cost_table(bqp_ctx* c) {
+  return c->launch.sort_by_cost == 1 ? (CostTable*)c->d_cost : nullptr;
+}
+
+// Adds the call's finished rows to the learned cost model (final-state calls
+// that use it).
+void enqueue_cost_learn(bqp_ctx* c, const PropParams& p, int64_t n) {
+  CostTable* tab = cost_table(c);
+  if (!tab || p.t_eval != nullptr || !p.tf || !p.t0) return;
+  hipLaunchKernelGGL(cost_learn_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, p.y0, p.t0,
+                     p.tf, p.status, p.evals, c->mu, c->radius, (int)n, tab);
+}
+
+// Forgets what the cost model learned (new field or configuration).
+int reset_cost_model(bqp_ctx* c) {
+  hipLaunchKernelGGL(cost_clear_kernel, dim3(1), dim3(256), 0, c->stream, (CostTable*)c->d_cost);
+  return hipGetLastError() == hipSuccess ? BQP_OK : BQP_E_DEVICE;
+}
+
 // Enqueues the whole call (order, queue reset, persistent kernel, row
 // statistics) on the context stream.  Pure kernel launches: no allocation,
 // no memset, no synchronisation, so the sequence can be captured in a graph.
@@ -267,9 +287,11 @@ void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, prop_fn ker
     int32_t* order = (int32_t*)(base + SORT_BUCKETS * sizeof(unsigned int));
     unsigned short* bucket = (unsigned short*)(base + SORT_BUCKETS * sizeof(unsigned int) + n * sizeof(int32_t));
     const unsigned nb = (unsigned)((n + 255) / 256);
+    CostTable* tab = cost_table(c);
+    if (tab) hipLaunchKernelGGL(cost_fill_kernel, dim3(1), dim3(512), 0, c->stream, tab);
     hipLaunchKernelGGL(sort_clear_kernel, dim3(1), dim3(1024), 0, c->stream, hist);
-    hipLaunchKernelGGL(sort_hist_kernel, dim3(nb), dim3(256), 0, c->stream, p.y0, p.t0, p.tf, c->mu, (int)n,
-                       hist, bucket);
+    hipLaunchKernelGGL(sort_hist_kernel, dim3(nb), dim3(256), 0, c->stream, p.y0, p.t0, p.tf, c->mu, c->radius,
+                       tab, (int)n, hist, bucket);
     hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(1024), 0, c->stream, hist);
     hipLaunchKernelGGL(sort_scatter_kernel, dim3(nb), dim3(256), 0, c->stream, bucket, (int)n, hist, order);
     q.order = order;
@@ -285,6 +307,7 @@ void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, prop_fn ker
                      c->stream, q.status, q.steps, q.evals, q.rejects, (int)n, c->d_totals);
   hipLaunchKernelGGL(first_failed_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, q.status,
                      (int)n, c->d_first);
+  enqueue_cost_learn(c, p, n);
 }
 
 // Algorithmic FLOPs of one FP64 gravity evaluation in the rescaled recursion
@@ -340,9 +363,11 @@ int run_surrogate_partitioned(bqp_ctx* c, PropParams p, int64_t n, int kprec, in
     order = (int32_t*)(base + SORT_BUCKETS * sizeof(unsigned int));
     unsigned short* bucket = (unsigned short*)(base + SORT_BUCKETS * sizeof(unsigned int) + n * sizeof(int32_t));
     const unsigned nbs = (unsigned)((n + 255) / 256);
+    CostTable* tab = cost_table(c);
+    if (tab) hipLaunchKernelGGL(cost_fill_kernel, dim3(1), dim3(512), 0, c->stream, tab);
     hipLaunchKernelGGL(sort_clear_kernel, dim3(1), dim3(1024), 0, c->stream, hist);
-    hipLaunchKernelGGL(sort_hist_kernel, dim3(nbs), dim3(256), 0, c->stream, p.y0, p.t0, p.tf, c->mu, (int)n,
-                       hist, bucket);
+    hipLaunchKernelGGL(sort_hist_kernel, dim3(nbs), dim3(256), 0, c->stream, p.y0, p.t0, p.tf, c->mu, c->radius,
+                       tab, (int)n, hist, bucket);
     hipLaunchKernelGGL(sort_scan_kernel, dim3(1), dim3(1024), 0, c->stream, hist);
     hipLaunchKernelGGL(sort_scatter_kernel, dim3(nbs), dim3(256), 0, c->stream, bucket, (int)n, hist, order);
   }
@@ -384,6 +409,7 @@ int run_surrogate_partitioned(bqp_ctx* c, PropParams p, int64_t n, int kprec, in
                      c->stream, p.status, p.steps, p.evals, p.rejects, (int)n, c->d_totals);
   hipLaunchKernelGGL(first_failed_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, p.status,
                      (int)n, c->d_first);
+  enqueue_cost_learn(c, p, n);
   HIP_TRY(c, hipGetLastError());
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   unsigned long long tot[5] = {0, 0, 0, 0, 0}, st[5] = {0, 0, 0, 0, 0};
@@ -518,7 +544,7 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
     key.cfg = cfg;
     key.G = G;
     key.blocks = blocks;
-    key.sort = sort;
+    key.sort = (sort ? 8 : 0) | c->launch.sort_by_cost;  // the mode decides the cost-model kernels
     key.tab = c->d_tab;
     if (!c->graph_key) {
       c->graph_key = new (std::nothrow) GraphKey();
@@ -642,7 +668,8 @@ int32_t bqp_create(const bqp_config* cfg, bqp_ctx** out) {
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
       hipMalloc(&c->d_queue, 256) != hipSuccess || hipMalloc(&c->d_totals, 256) != hipSuccess ||
-      hipMalloc(&c->d_first, 256) != hipSuccess) {
+      hipMalloc(&c->d_first, 256) != hipSuccess || hipMalloc(&c->d_cost, sizeof(CostTable)) != hipSuccess ||
+      reset_cost_model(c) != BQP_OK) {
     bqp_destroy(c);
     return BQP_E_DEVICE;
   }
@@ -684,6 +711,7 @@ int32_t bqp_destroy(bqp_ctx* c) {
   (void)hipFree(c->d_stage);
   (void)hipFree(c->d_sort);
   (void)hipFree(c->d_part);
+  (void)hipFree(c->d_cost);
   bqp_mlp_release(c);
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -698,7 +726,9 @@ int32_t bqp_set_config(bqp_ctx* c, const bqp_config* cfg) {
   if (!c || !cfg) return BQP_E_INVALID;
   std::string why;
   if (!valid_config(*cfg, why)) return fail(c, BQP_E_INVALID, why);
+  const bool changed = std::memcmp(&c->cfg, cfg, sizeof(bqp_config)) != 0;
   c->cfg = *cfg;
+  if (changed && reset_cost_model(c) != BQP_OK) return fail(c, BQP_E_DEVICE, "cost model reset failed");
   return BQP_OK;
 }
 
@@ -709,11 +739,29 @@ int32_t bqp_set_launch(bqp_ctx* c, const bqp_launch* l) {
     return fail(c, BQP_E_INVALID, "waves_per_group must be 0 (auto), 1, 2, 4, 8 or 16");
   if (l->blocks < 0) return fail(c, BQP_E_INVALID, "blocks must be >= 0");
   if (l->use_graph != 0 && l->use_graph != 1) return fail(c, BQP_E_INVALID, "use_graph must be 0 or 1");
+  if (l->sort_by_cost < 0 || l->sort_by_cost > 2) return fail(c, BQP_E_INVALID, "sort_by_cost must be 0, 1 or 2");
   c->launch = *l;
   return BQP_OK;
 }
 
 const char* bqp_last_error(const bqp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int32_t bqp_get_cost_model(bqp_ctx* c, int64_t* evals, double* revolutions, int64_t* rows) {
+  if (!c || !c->d_cost) return BQP_E_INVALID;
+  std::vector<unsigned long long> e(COST_CELLS), r(COST_CELLS);
+  std::vector<unsigned int> k(COST_CELLS);
+  const CostTable* t = (const CostTable*)c->d_cost;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  HIP_TRY(c, hipMemcpy(e.data(), t->evals, COST_CELLS * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(r.data(), t->revs, COST_CELLS * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(k.data(), t->rows, COST_CELLS * sizeof(unsigned int), hipMemcpyDeviceToHost));
+  for (int i = 0; i < COST_CELLS; ++i) {
+    if (evals) evals[i] = (int64_t)e[i];
+    if (revolutions) revolutions[i] = (double)r[i] / COST_REV_UNIT;
+    if (rows) rows[i] = (int64_t)k[i];
+  }
+  return BQP_OK;
+}
 
 int32_t bqp_load_field(bqp_ctx* c, int32_t nmax, int32_t mmax, const double* cbar, const double* sbar,
                        double mu, double radius) {
@@ -782,6 +830,7 @@ int32_t bqp_load_field(bqp_ctx* c, int32_t nmax, int32_t mmax, const double* cba
   c->jmax = jmax;
   c->mu = mu;
   c->radius = radius;
+  if (reset_cost_model(c) != BQP_OK) return fail(c, BQP_E_DEVICE, "cost model reset failed");
   return BQP_OK;
 }
 

@@ -158,26 +158,149 @@ __global__ void __launch_bounds__(1024) partition_kernel(const int* __restrict__
 __global__ void reset_counter_kernel(unsigned int* q) { q[0] = 0u; }
 
 // Longest-first hand-out order (a counting sort made of plain kernels, so it
-// can be captured in a HIP graph).  The cost key is the number of orbit
-// revolutions over the span (span * mean motion from vis-viva); at a fixed
-// tolerance the step count per revolution varies slowly, so rows with similar
-// keys take similar step counts and share a wave.  Row results do not depend
-// on which lane runs them, so any tie order is exact.
+// can be captured in a HIP graph).  The key estimates a row's evaluations as
+// revolutions over the span (span * mean motion from vis-viva) times
+// evaluations per revolution.  The latter come from a table that every call
+// on the context fills from its own per-row counts (CostTable below); with
+// no table, or an untrained one, the key is the revolution count.  Row
+// results do not depend on which lane runs them, so any order, and any tie
+// order, gives the same results.
 constexpr int SORT_BUCKETS = 4096;
 
-__device__ __forceinline__ int cost_bucket(const double* __restrict__ s, double span, double mu) {
+// Learned evaluations per revolution over (altitude of the semi-major axis,
+// sin^2 of the inclination): COST_NA log-spaced altitude bins over
+// [COST_ALT0, COST_ALT0 * 10^COST_DECADES] and COST_NI bins of sin^2 i.  The
+// cost per revolution rises steeply as altitude falls (the high-degree terms
+// grow as (R/r)^n) and by a few percent from equatorial to polar orbits; the
+// table captures both without a model of either.  Sums are integers (evals,
+// revolutions in 1/65536 units), so the learned values do not depend on the
+// order of the atomic additions.
+constexpr int COST_NA = BQP_COST_ALT_BINS, COST_NI = BQP_COST_INC_BINS, COST_CELLS = COST_NA * COST_NI;
+constexpr double COST_ALT0 = 1e5, COST_DECADES = 3.0, COST_REV_UNIT = 65536.0;
+constexpr unsigned int COST_MIN_ROWS = 4;
+
+struct CostTable {
+  unsigned long long evals[COST_CELLS];
+  unsigned long long revs[COST_CELLS];  // revolutions * COST_REV_UNIT
+  unsigned int rows[COST_CELLS];
+  float log2_epr[COST_CELLS];  // filled log2(evals per revolution), read by the key
+};
+
+// Revolutions over the span and the row's (continuous) altitude-bin position
+// and inclination bin; false for unbound or degenerate states.
+__device__ __forceinline__ bool row_orbit(const double* __restrict__ s, double span, double mu, double radius,
+                                          double* rev, double* ualt, int* ii) {
   const double r = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
   const double v2 = s[3] * s[3] + s[4] * s[4] + s[5] * s[5];
   const double en = 0.5 * v2 - mu / r;
-  double rev = 0.0;
-  if (en < 0.0 && r > 0.0 && span > 0.0) {
-    const double a = -mu / (2.0 * en);
-    rev = span * sqrt(mu / (a * a * a));
+  if (!(en < 0.0) || !(r > 0.0) || !(span > 0.0)) return false;
+  const double a = -mu / (2.0 * en);
+  *rev = span * sqrt(mu / (a * a * a));
+  if (!isfinite(*rev) || !(*rev > 0.0)) return false;
+  const double alt = fmax(a - radius, COST_ALT0);
+  *ualt = log10(alt / COST_ALT0) * (COST_NA / COST_DECADES);
+  const double hx = s[1] * s[5] - s[2] * s[4], hy = s[2] * s[3] - s[0] * s[5], hz = s[0] * s[4] - s[1] * s[3];
+  const double h2 = hx * hx + hy * hy + hz * hz;
+  const double s2 = h2 > 0.0 ? (hx * hx + hy * hy) / h2 : 0.0;
+  const int b = (int)(s2 * COST_NI);
+  *ii = b < 0 ? 0 : (b > COST_NI - 1 ? COST_NI - 1 : b);
+  return true;
+}
+
+__device__ __forceinline__ int cost_cell(double ualt, int ii) {
+  int ia = isfinite(ualt) ? (int)ualt : COST_NA - 1;
+  ia = ia < 0 ? 0 : (ia > COST_NA - 1 ? COST_NA - 1 : ia);
+  return ia * COST_NI + ii;
+}
+
+__device__ __forceinline__ int cost_bucket(const double* __restrict__ s, double span, double mu, double radius,
+                                           const CostTable* __restrict__ tab) {
+  double rev = 0.0, ualt = 0.0;
+  int ii = 0;
+  double est = 0.0;
+  if (row_orbit(s, span, mu, radius, &rev, &ualt, &ii)) {
+    est = rev;
+    if (tab) {
+      // linear in log altitude between bin centres
+      const double uc = (isfinite(ualt) ? ualt : (double)COST_NA) - 0.5;
+      int i0 = (int)floor(uc);
+      i0 = i0 < 0 ? 0 : (i0 > COST_NA - 1 ? COST_NA - 1 : i0);
+      const int i1 = i0 + 1 < COST_NA ? i0 + 1 : COST_NA - 1;
+      double f = uc - (double)i0;
+      f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+      const double l = (1.0 - f) * (double)tab->log2_epr[i0 * COST_NI + ii] + f * (double)tab->log2_epr[i1 * COST_NI + ii];
+      est = rev * exp2(l);
+    }
   }
-  // 128 buckets per octave of revolutions, most expensive first
-  int b = isfinite(rev) && rev > 0.0 ? (int)(log2(1.0 + rev) * 128.0) : 0;
+  // 128 buckets per octave of the estimate, most expensive first
+  int b = isfinite(est) && est > 0.0 ? (int)(log2(1.0 + est) * 128.0) : 0;
   b = b < 0 ? 0 : (b > SORT_BUCKETS - 1 ? SORT_BUCKETS - 1 : b);
   return SORT_BUCKETS - 1 - b;
+}
+
+__global__ void cost_clear_kernel(CostTable* __restrict__ tab) {
+  for (int i = threadIdx.x; i < COST_CELLS; i += blockDim.x) {
+    tab->evals[i] = 0ull;
+    tab->revs[i] = 0ull;
+    tab->rows[i] = 0u;
+    tab->log2_epr[i] = 0.0f;
+  }
+}
+
+// Adds every finished row's evaluations and revolutions to its cell.
+__global__ void cost_learn_kernel(const double* __restrict__ y0, const double* __restrict__ t0,
+                                  const double* __restrict__ tf, const int32_t* __restrict__ status,
+                                  const int32_t* __restrict__ evals, double mu, double radius, int n,
+                                  CostTable* __restrict__ tab) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || status[i] != ROW_OK || evals[i] <= 0) return;
+  double rev = 0.0, ualt = 0.0;
+  int ii = 0;
+  if (!row_orbit(y0 + 6 * (size_t)i, tf[i] - t0[i], mu, radius, &rev, &ualt, &ii)) return;
+  const double rq = rev * COST_REV_UNIT;
+  if (!(rq >= 1.0) || rq > 1e15) return;
+  const int cell = cost_cell(ualt, ii);
+  atomicAdd(&tab->evals[cell], (unsigned long long)evals[i]);
+  atomicAdd(&tab->revs[cell], (unsigned long long)(rq + 0.5));
+  atomicAdd(&tab->rows[cell], 1u);
+}
+
+// Evaluations per revolution of every cell with at least COST_MIN_ROWS rows;
+// an untrained cell takes the nearest trained cell in altitude within its
+// inclination bin, else in the nearest inclination bin that has one; an
+// untrained table is 1 evaluation per revolution.  One block.
+__global__ void __launch_bounds__(512) cost_fill_kernel(CostTable* __restrict__ tab) {
+  __shared__ float val[COST_CELLS];
+  __shared__ int have[COST_CELLS];
+  const int t = threadIdx.x;
+  if (t < COST_CELLS) {
+    const bool ok = tab->rows[t] >= COST_MIN_ROWS && tab->revs[t] > 0ull;
+    have[t] = ok ? 1 : 0;
+    val[t] = ok ? (float)log2((double)tab->evals[t] * COST_REV_UNIT / (double)tab->revs[t]) : 0.0f;
+  }
+  __syncthreads();
+  if (t < COST_CELLS) {
+    const int ia = t / COST_NI, ii = t % COST_NI;
+    float v = 0.0f;
+    bool found = false;
+    for (int di = 0; di < COST_NI && !found; ++di) {
+      for (int sgn = 0; sgn < 2 && !found; ++sgn) {
+        const int jj = sgn ? ii + di : ii - di;
+        if (jj < 0 || jj >= COST_NI || (sgn && di == 0)) continue;
+        for (int da = 0; da < COST_NA && !found; ++da) {
+          const int lo = ia - da, hi = ia + da;
+          if (lo >= 0 && have[lo * COST_NI + jj]) {
+            v = val[lo * COST_NI + jj];
+            found = true;
+          } else if (hi < COST_NA && have[hi * COST_NI + jj]) {
+            v = val[hi * COST_NI + jj];
+            found = true;
+          }
+        }
+      }
+    }
+    tab->log2_epr[t] = v;
+  }
 }
 
 __global__ void sort_clear_kernel(unsigned int* __restrict__ hist) {
@@ -185,11 +308,12 @@ __global__ void sort_clear_kernel(unsigned int* __restrict__ hist) {
 }
 
 __global__ void sort_hist_kernel(const double* __restrict__ y0, const double* __restrict__ t0,
-                                 const double* __restrict__ tf, double mu, int n,
-                                 unsigned int* __restrict__ hist, unsigned short* __restrict__ bucket) {
+                                 const double* __restrict__ tf, double mu, double radius,
+                                 const CostTable* __restrict__ tab, int n, unsigned int* __restrict__ hist,
+                                 unsigned short* __restrict__ bucket) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int b = cost_bucket(y0 + 6 * (size_t)i, tf[i] - t0[i], mu);
+  const int b = cost_bucket(y0 + 6 * (size_t)i, tf[i] - t0[i], mu, radius, tab);
   bucket[i] = (unsigned short)b;
   atomicAdd(&hist[b], 1u);
 }

@@ -66,6 +66,7 @@ struct bqp_ctx {
   void* d_sort = nullptr;
   size_t sort_bytes = 0;
   void* d_part = nullptr;  // surrogate up-front check: flags and the two row lists
+  void* d_cost = nullptr;  // learned cost model (bqp::CostTable), allocated with the context
   size_t part_bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipGraphExec_t graph_exec = nullptr;

@@ -125,11 +125,27 @@ class Propagator:
 
     def set_launch(self, waves_per_group: int = 0, use_graph: int = 0, blocks: int = 0, sort_by_cost: int = 1):
         """Launch shape: waves per trajectory group G (0 = auto), HIP-graph
-        replay, persistent grid size (0 = fill the device), cost-sorted hand-out."""
+        replay, persistent grid size (0 = fill the device), hand-out order
+        (0 input order, 1 longest first by the learned cost model, 2 longest
+        first by revolutions)."""
         L = nat.Launch(waves_per_group, use_graph, blocks, sort_by_cost)
         self._check(self._lib.bqp_set_launch(self._ctx, C.byref(L)))
         self.launch = {"waves_per_group": waves_per_group, "use_graph": use_graph, "blocks": blocks,
                        "sort_by_cost": sort_by_cost}
+
+    def cost_model(self) -> dict:
+        """The learned cost model (sort_by_cost = 1): per (altitude bin,
+        sin^2 i bin) cell, the evaluations, revolutions and rows summed over
+        the rows that finished ok since the field or configuration was set."""
+        cells = nat.COST_ALT_BINS * nat.COST_INC_BINS
+        ev = np.zeros(cells, dtype=np.int64)
+        rev = np.zeros(cells)
+        rows = np.zeros(cells, dtype=np.int64)
+        self._check(self._lib.bqp_get_cost_model(self._ctx, ev.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                 rev.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 rows.ctypes.data_as(C.POINTER(C.c_int64))))
+        shape = (nat.COST_ALT_BINS, nat.COST_INC_BINS)
+        return {"evals": ev.reshape(shape), "revolutions": rev.reshape(shape), "rows": rows.reshape(shape)}
 
     def reconfigure(self, **kw):
         """Change integrator/model settings (same keywords as the constructor)."""

@@ -122,8 +122,20 @@ typedef struct bqp_launch {
   int32_t use_graph;        /* 1 = capture the call's launch sequence in a HIP graph once and
                                replay it while buffers, sizes and settings are unchanged */
   int32_t blocks;           /* persistent grid size; 0 = fill the device */
-  int32_t sort_by_cost;     /* 1 = hand out rows in descending estimated cost */
+  int32_t sort_by_cost;     /* hand-out order of the rows (final-state calls):
+                               0 = input order;
+                               1 = descending estimated evaluations, the estimate being
+                                   revolutions x evaluations per revolution learned from
+                                   the context's earlier calls (bqp_get_cost_model);
+                               2 = descending revolutions (nothing learned) */
 } bqp_launch;
+
+/* Cells of the learned cost model: BQP_COST_ALT_BINS log-spaced bins of the
+ * altitude of the semi-major axis above the field radius, 100 km to
+ * 100,000 km, times BQP_COST_INC_BINS equal bins of sin^2(inclination);
+ * cell = alt_bin * BQP_COST_INC_BINS + inc_bin. */
+#define BQP_COST_ALT_BINS 96
+#define BQP_COST_INC_BINS 4
 
 int32_t bqp_default_config(bqp_config* out);
 int32_t bqp_create(const bqp_config* cfg, bqp_ctx** out);
@@ -171,6 +183,13 @@ int32_t bqp_get_tableau(int32_t method, bqp_tableau* out);
 
 /* Algorithmic FLOPs of one RHS evaluation at the loaded field / config. */
 double bqp_flops_per_rhs(const bqp_ctx* ctx);
+
+/* The learned cost model behind sort_by_cost = 1 (no counterpart in the
+ * reference, which runs rows in input order): per cell, the evaluations,
+ * revolutions and row count of every row that finished ok in final-state
+ * calls since the field or configuration was last set.  Each array holds
+ * BQP_COST_ALT_BINS * BQP_COST_INC_BINS values; any may be NULL. */
+int32_t bqp_get_cost_model(bqp_ctx* ctx, int64_t* evals, double* revolutions, int64_t* rows);
 
 /* ------------------------------------------------------------------------
  * Learned gravity surrogate.  No counterpart in the reference (which always

@@ -149,8 +149,16 @@ def test_launch_shape_bit_identity(G):
         got, s_got, _ = p.propagate_batch_final(w["y0"][:700], 0.0, tf)
         p.set_launch(waves_per_group=1, sort_by_cost=1)
         g1, _, _ = p.propagate_batch_final(w["y0"][:700], 0.0, tf)
-    assert np.array_equal(s_ref, s_got)
+        # the learned cost model now orders the rows differently
+        p.set_launch(waves_per_group=G, sort_by_cost=1)
+        learned, s_learned, _ = p.propagate_batch_final(w["y0"][:700], 0.0, tf)
+        p.set_launch(waves_per_group=G, sort_by_cost=2)
+        by_rev, _, _ = p.propagate_batch_final(w["y0"][:700], 0.0, tf)
+        assert p.cost_model()["rows"].sum() == 3 * 700  # the three calls with sort_by_cost = 1
+    assert np.array_equal(s_ref, s_got) and np.array_equal(s_ref, s_learned)
     assert np.array_equal(ref, got)
+    assert np.array_equal(ref, learned)
+    assert np.array_equal(ref, by_rev)
     r = np.linalg.norm(ref[:, :3], axis=1)
     assert np.all(pos_err(ref, g1) <= 10 * 1e-11 * r)
 
