@@ -749,12 +749,12 @@ const char* bqp_last_error(const bqp_ctx* c) { return c ? c->err.c_str() : "null
 int32_t bqp_get_cost_model(bqp_ctx* c, int64_t* evals, double* revolutions, int64_t* rows) {
   if (!c || !c->d_cost) return BQP_E_INVALID;
   std::vector<unsigned long long> e(COST_CELLS), r(COST_CELLS);
-  std::vector<unsigned int> k(COST_CELLS);
+  std::vector<unsigned long long> k(COST_CELLS);
   const CostTable* t = (const CostTable*)c->d_cost;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   HIP_TRY(c, hipMemcpy(e.data(), t->evals, COST_CELLS * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   HIP_TRY(c, hipMemcpy(r.data(), t->revs, COST_CELLS * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-  HIP_TRY(c, hipMemcpy(k.data(), t->rows, COST_CELLS * sizeof(unsigned int), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(k.data(), t->rows, COST_CELLS * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   for (int i = 0; i < COST_CELLS; ++i) {
     if (evals) evals[i] = (int64_t)e[i];
     if (revolutions) revolutions[i] = (double)r[i] / COST_REV_UNIT;

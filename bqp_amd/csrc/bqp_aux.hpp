@@ -177,12 +177,12 @@ constexpr int SORT_BUCKETS = 4096;
 // order of the atomic additions.
 constexpr int COST_NA = BQP_COST_ALT_BINS, COST_NI = BQP_COST_INC_BINS, COST_CELLS = COST_NA * COST_NI;
 constexpr double COST_ALT0 = 1e5, COST_DECADES = 3.0, COST_REV_UNIT = 65536.0;
-constexpr unsigned int COST_MIN_ROWS = 4;
+constexpr unsigned long long COST_MIN_ROWS = 4;
 
 struct CostTable {
   unsigned long long evals[COST_CELLS];
   unsigned long long revs[COST_CELLS];  // revolutions * COST_REV_UNIT
-  unsigned int rows[COST_CELLS];
+  unsigned long long rows[COST_CELLS];
   float log2_epr[COST_CELLS];  // filled log2(evals per revolution), read by the key
 };
 
@@ -242,12 +242,14 @@ __global__ void cost_clear_kernel(CostTable* __restrict__ tab) {
   for (int i = threadIdx.x; i < COST_CELLS; i += blockDim.x) {
     tab->evals[i] = 0ull;
     tab->revs[i] = 0ull;
-    tab->rows[i] = 0u;
+    tab->rows[i] = 0ull;
     tab->log2_epr[i] = 0.0f;
   }
 }
 
-// Adds every finished row's evaluations and revolutions to its cell.
+// Adds every finished row of at least one revolution to its cell (a shorter
+// row's fixed start-up cost, the first step's selection, would inflate its
+// evaluations per revolution).
 __global__ void cost_learn_kernel(const double* __restrict__ y0, const double* __restrict__ t0,
                                   const double* __restrict__ tf, const int32_t* __restrict__ status,
                                   const int32_t* __restrict__ evals, double mu, double radius, int n,
@@ -257,12 +259,12 @@ __global__ void cost_learn_kernel(const double* __restrict__ y0, const double* _
   double rev = 0.0, ualt = 0.0;
   int ii = 0;
   if (!row_orbit(y0 + 6 * (size_t)i, tf[i] - t0[i], mu, radius, &rev, &ualt, &ii)) return;
+  if (!(rev >= 1.0) || rev > 1e9) return;
   const double rq = rev * COST_REV_UNIT;
-  if (!(rq >= 1.0) || rq > 1e15) return;
   const int cell = cost_cell(ualt, ii);
   atomicAdd(&tab->evals[cell], (unsigned long long)evals[i]);
   atomicAdd(&tab->revs[cell], (unsigned long long)(rq + 0.5));
-  atomicAdd(&tab->rows[cell], 1u);
+  atomicAdd(&tab->rows[cell], 1ull);
 }
 
 // Evaluations per revolution of every cell with at least COST_MIN_ROWS rows;

@@ -186,8 +186,9 @@ double bqp_flops_per_rhs(const bqp_ctx* ctx);
 
 /* The learned cost model behind sort_by_cost = 1 (no counterpart in the
  * reference, which runs rows in input order): per cell, the evaluations,
- * revolutions and row count of every row that finished ok in final-state
- * calls since the field or configuration was last set.  Each array holds
+ * revolutions and row count of every row of at least one revolution that
+ * finished ok in final-state calls with sort_by_cost = 1 since the field or
+ * configuration was last set.  Each array holds
  * BQP_COST_ALT_BINS * BQP_COST_INC_BINS values; any may be NULL. */
 int32_t bqp_get_cost_model(bqp_ctx* ctx, int64_t* evals, double* revolutions, int64_t* rows);
 
