@@ -133,6 +133,14 @@ def test_header_declares_exactly_the_bound_symbols():
     assert _header_functions() == sorted(_native.EXPORTS)
 
 
+def test_cost_model_cells_match_the_header():
+    with open(os.path.join(ROOT, "include", "bqp.h")) as fh:
+        txt = fh.read()
+    defines = dict(re.findall(r"#define (BQP_COST_[A-Z_]+) (\d+)", txt))
+    assert int(defines["BQP_COST_ALT_BINS"]) == _native.COST_ALT_BINS
+    assert int(defines["BQP_COST_INC_BINS"]) == _native.COST_INC_BINS
+
+
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libbqp.so not built")
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(LIB)
@@ -153,6 +161,7 @@ def test_host_only_entry_points_without_gpu():
     assert L.bqp_propagate_batch_final(None, None, None, None, None, 0, None, None, None, None, None) == \
         _native.E_INVALID
     assert L.bqp_last_error(None) == b"null context"
+    assert L.bqp_get_cost_model(None, None, None, None) == _native.E_INVALID
 
 
 def test_sampled_call_returns_only_the_samples_a_failed_arc_reached():
