@@ -25,7 +25,7 @@ ROW_STATUS = {
 }
 # cells of the learned cost model (include/bqp.h BQP_COST_ALT_BINS, BQP_COST_INC_BINS)
 COST_ALT_BINS = 96
-COST_INC_BINS = 4
+COST_INC_BINS = 16
 
 METHODS = {"dopri5": 0, "dop853": 1, "rk87": 2, "dopri87": 2, "rk8(7)13m": 2}
 CONTROLS = {"scaled": 0, "absolute": 1, "compat": 2}

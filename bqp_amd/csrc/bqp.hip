@@ -288,7 +288,7 @@ void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, prop_fn ker
     unsigned short* bucket = (unsigned short*)(base + SORT_BUCKETS * sizeof(unsigned int) + n * sizeof(int32_t));
     const unsigned nb = (unsigned)((n + 255) / 256);
     CostTable* tab = cost_table(c);
-    if (tab) hipLaunchKernelGGL(cost_fill_kernel, dim3(1), dim3(512), 0, c->stream, tab);
+    if (tab) hipLaunchKernelGGL(cost_fill_kernel, dim3(1), dim3(1024), 0, c->stream, tab);
     hipLaunchKernelGGL(sort_clear_kernel, dim3(1), dim3(1024), 0, c->stream, hist);
     hipLaunchKernelGGL(sort_hist_kernel, dim3(nb), dim3(256), 0, c->stream, p.y0, p.t0, p.tf, c->mu, c->radius,
                        tab, (int)n, hist, bucket);
@@ -364,7 +364,7 @@ int run_surrogate_partitioned(bqp_ctx* c, PropParams p, int64_t n, int kprec, in
     unsigned short* bucket = (unsigned short*)(base + SORT_BUCKETS * sizeof(unsigned int) + n * sizeof(int32_t));
     const unsigned nbs = (unsigned)((n + 255) / 256);
     CostTable* tab = cost_table(c);
-    if (tab) hipLaunchKernelGGL(cost_fill_kernel, dim3(1), dim3(512), 0, c->stream, tab);
+    if (tab) hipLaunchKernelGGL(cost_fill_kernel, dim3(1), dim3(1024), 0, c->stream, tab);
     hipLaunchKernelGGL(sort_clear_kernel, dim3(1), dim3(1024), 0, c->stream, hist);
     hipLaunchKernelGGL(sort_hist_kernel, dim3(nbs), dim3(256), 0, c->stream, p.y0, p.t0, p.tf, c->mu, c->radius,
                        tab, (int)n, hist, bucket);

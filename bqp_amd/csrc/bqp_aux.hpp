@@ -168,13 +168,14 @@ __global__ void reset_counter_kernel(unsigned int* q) { q[0] = 0u; }
 constexpr int SORT_BUCKETS = 4096;
 
 // Learned evaluations per revolution over (altitude of the semi-major axis,
-// sin^2 of the inclination): COST_NA log-spaced altitude bins over
-// [COST_ALT0, COST_ALT0 * 10^COST_DECADES] and COST_NI bins of sin^2 i.  The
-// cost per revolution rises steeply as altitude falls (the high-degree terms
-// grow as (R/r)^n) and by a few percent from equatorial to polar orbits; the
-// table captures both without a model of either.  Sums are integers (evals,
-// revolutions in 1/65536 units), so the learned values do not depend on the
-// order of the atomic additions.
+// cos of the inclination): COST_NA log-spaced altitude bins over
+// [COST_ALT0, COST_ALT0 * 10^COST_DECADES] and COST_NI equal bins of cos i
+// over [-1, 1].  The cost per revolution rises steeply as altitude falls (the
+// high-degree terms grow as (R/r)^n) and by several percent from equatorial
+// to polar orbits; the table captures both without a model of either, and
+// the key interpolates it bilinearly between cell centres.  Sums are
+// integers (evals, revolutions in 1/65536 units), so the learned values do
+// not depend on the order of the atomic additions.
 constexpr int COST_NA = BQP_COST_ALT_BINS, COST_NI = BQP_COST_INC_BINS, COST_CELLS = COST_NA * COST_NI;
 constexpr double COST_ALT0 = 1e5, COST_DECADES = 3.0, COST_REV_UNIT = 65536.0;
 constexpr unsigned long long COST_MIN_ROWS = 4;
@@ -186,10 +187,10 @@ struct CostTable {
   float log2_epr[COST_CELLS];  // filled log2(evals per revolution), read by the key
 };
 
-// Revolutions over the span and the row's (continuous) altitude-bin position
-// and inclination bin; false for unbound or degenerate states.
+// Revolutions over the span and the row's continuous bin positions (altitude
+// and inclination, in units of bins); false for unbound or degenerate states.
 __device__ __forceinline__ bool row_orbit(const double* __restrict__ s, double span, double mu, double radius,
-                                          double* rev, double* ualt, int* ii) {
+                                          double* rev, double* ualt, double* uinc) {
   const double r = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
   const double v2 = s[3] * s[3] + s[4] * s[4] + s[5] * s[5];
   const double en = 0.5 * v2 - mu / r;
@@ -200,36 +201,47 @@ __device__ __forceinline__ bool row_orbit(const double* __restrict__ s, double s
   const double alt = fmax(a - radius, COST_ALT0);
   *ualt = log10(alt / COST_ALT0) * (COST_NA / COST_DECADES);
   const double hx = s[1] * s[5] - s[2] * s[4], hy = s[2] * s[3] - s[0] * s[5], hz = s[0] * s[4] - s[1] * s[3];
-  const double h2 = hx * hx + hy * hy + hz * hz;
-  const double s2 = h2 > 0.0 ? (hx * hx + hy * hy) / h2 : 0.0;
-  const int b = (int)(s2 * COST_NI);
-  *ii = b < 0 ? 0 : (b > COST_NI - 1 ? COST_NI - 1 : b);
-  return true;
+  const double hn = sqrt(hx * hx + hy * hy + hz * hz);
+  const double ci = hn > 0.0 ? hz / hn : 1.0;
+  *uinc = 0.5 * (ci + 1.0) * COST_NI;
+  return isfinite(*ualt) && isfinite(*uinc);
 }
 
-__device__ __forceinline__ int cost_cell(double ualt, int ii) {
-  int ia = isfinite(ualt) ? (int)ualt : COST_NA - 1;
-  ia = ia < 0 ? 0 : (ia > COST_NA - 1 ? COST_NA - 1 : ia);
-  return ia * COST_NI + ii;
+__device__ __forceinline__ int cost_bin(double u, int nb) {
+  const int b = (int)u;
+  return b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+}
+
+__device__ __forceinline__ int cost_cell(double ualt, double uinc) {
+  return cost_bin(ualt, COST_NA) * COST_NI + cost_bin(uinc, COST_NI);
+}
+
+// Lower cell-centre index and weight of the upper one along one axis.
+__device__ __forceinline__ void cost_lerp(double u, int nb, int* i0, int* i1, double* f) {
+  const double uc = u - 0.5;
+  int k = (int)floor(uc);
+  k = k < 0 ? 0 : (k > nb - 1 ? nb - 1 : k);
+  *i0 = k;
+  *i1 = k + 1 < nb ? k + 1 : nb - 1;
+  const double w = uc - (double)k;
+  *f = w < 0.0 ? 0.0 : (w > 1.0 ? 1.0 : w);
 }
 
 __device__ __forceinline__ int cost_bucket(const double* __restrict__ s, double span, double mu, double radius,
                                            const CostTable* __restrict__ tab) {
-  double rev = 0.0, ualt = 0.0;
-  int ii = 0;
+  double rev = 0.0, ualt = 0.0, uinc = 0.0;
   double est = 0.0;
-  if (row_orbit(s, span, mu, radius, &rev, &ualt, &ii)) {
+  if (row_orbit(s, span, mu, radius, &rev, &ualt, &uinc)) {
     est = rev;
     if (tab) {
-      // linear in log altitude between bin centres
-      const double uc = (isfinite(ualt) ? ualt : (double)COST_NA) - 0.5;
-      int i0 = (int)floor(uc);
-      i0 = i0 < 0 ? 0 : (i0 > COST_NA - 1 ? COST_NA - 1 : i0);
-      const int i1 = i0 + 1 < COST_NA ? i0 + 1 : COST_NA - 1;
-      double f = uc - (double)i0;
-      f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
-      const double l = (1.0 - f) * (double)tab->log2_epr[i0 * COST_NI + ii] + f * (double)tab->log2_epr[i1 * COST_NI + ii];
-      est = rev * exp2(l);
+      int a0, a1, b0, b1;
+      double fa, fb;
+      cost_lerp(ualt, COST_NA, &a0, &a1, &fa);
+      cost_lerp(uinc, COST_NI, &b0, &b1, &fb);
+      const float* L = tab->log2_epr;
+      const double lo = (1.0 - fa) * (double)L[a0 * COST_NI + b0] + fa * (double)L[a1 * COST_NI + b0];
+      const double hi = (1.0 - fa) * (double)L[a0 * COST_NI + b1] + fa * (double)L[a1 * COST_NI + b1];
+      est = rev * exp2((1.0 - fb) * lo + fb * hi);
     }
   }
   // 128 buckets per octave of the estimate, most expensive first
@@ -256,48 +268,73 @@ __global__ void cost_learn_kernel(const double* __restrict__ y0, const double* _
                                   CostTable* __restrict__ tab) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n || status[i] != ROW_OK || evals[i] <= 0) return;
-  double rev = 0.0, ualt = 0.0;
-  int ii = 0;
-  if (!row_orbit(y0 + 6 * (size_t)i, tf[i] - t0[i], mu, radius, &rev, &ualt, &ii)) return;
+  double rev = 0.0, ualt = 0.0, uinc = 0.0;
+  if (!row_orbit(y0 + 6 * (size_t)i, tf[i] - t0[i], mu, radius, &rev, &ualt, &uinc)) return;
   if (!(rev >= 1.0) || rev > 1e9) return;
   const double rq = rev * COST_REV_UNIT;
-  const int cell = cost_cell(ualt, ii);
+  const int cell = cost_cell(ualt, uinc);
   atomicAdd(&tab->evals[cell], (unsigned long long)evals[i]);
   atomicAdd(&tab->revs[cell], (unsigned long long)(rq + 0.5));
   atomicAdd(&tab->rows[cell], 1ull);
 }
 
-// Evaluations per revolution of every cell with at least COST_MIN_ROWS rows;
-// an untrained cell takes the nearest trained cell in altitude within its
-// inclination bin, else in the nearest inclination bin that has one; an
+// Evaluations per revolution of every cell with at least COST_MIN_ROWS rows.
+// An untrained cell takes the nearest trained cell in altitude within its
+// inclination bin; in an inclination bin with no trained cell, cells take
+// the value at the same altitude in the nearest bin that has one.  An
 // untrained table is 1 evaluation per revolution.  One block.
-__global__ void __launch_bounds__(512) cost_fill_kernel(CostTable* __restrict__ tab) {
+__global__ void __launch_bounds__(1024) cost_fill_kernel(CostTable* __restrict__ tab) {
   __shared__ float val[COST_CELLS];
   __shared__ int have[COST_CELLS];
-  const int t = threadIdx.x;
-  if (t < COST_CELLS) {
+  __shared__ int col_have[COST_NI];
+  for (int t = threadIdx.x; t < COST_CELLS; t += blockDim.x) {
     const bool ok = tab->rows[t] >= COST_MIN_ROWS && tab->revs[t] > 0ull;
     have[t] = ok ? 1 : 0;
     val[t] = ok ? (float)log2((double)tab->evals[t] * COST_REV_UNIT / (double)tab->revs[t]) : 0.0f;
   }
+  for (int t = threadIdx.x; t < COST_NI; t += blockDim.x) col_have[t] = 0;
   __syncthreads();
-  if (t < COST_CELLS) {
+  for (int t = threadIdx.x; t < COST_CELLS; t += blockDim.x)
+    if (have[t]) col_have[t % COST_NI] = 1;  // benign race: every writer stores 1
+  __syncthreads();
+  // along altitude within the inclination bin
+  float v1[(COST_CELLS + 255) / 256];  // launched with 256 to 1024 threads
+  int k = 0;
+  for (int t = threadIdx.x; t < COST_CELLS; t += blockDim.x, ++k) {
     const int ia = t / COST_NI, ii = t % COST_NI;
-    float v = 0.0f;
-    bool found = false;
-    for (int di = 0; di < COST_NI && !found; ++di) {
-      for (int sgn = 0; sgn < 2 && !found; ++sgn) {
-        const int jj = sgn ? ii + di : ii - di;
-        if (jj < 0 || jj >= COST_NI || (sgn && di == 0)) continue;
-        for (int da = 0; da < COST_NA && !found; ++da) {
-          const int lo = ia - da, hi = ia + da;
-          if (lo >= 0 && have[lo * COST_NI + jj]) {
-            v = val[lo * COST_NI + jj];
-            found = true;
-          } else if (hi < COST_NA && have[hi * COST_NI + jj]) {
-            v = val[hi * COST_NI + jj];
-            found = true;
-          }
+    float v = val[t];
+    if (!have[t] && col_have[ii]) {
+      for (int d = 1; d < COST_NA; ++d) {
+        if (ia - d >= 0 && have[(ia - d) * COST_NI + ii]) {
+          v = val[(ia - d) * COST_NI + ii];
+          break;
+        }
+        if (ia + d < COST_NA && have[(ia + d) * COST_NI + ii]) {
+          v = val[(ia + d) * COST_NI + ii];
+          break;
+        }
+      }
+    }
+    v1[k] = v;
+  }
+  __syncthreads();
+  k = 0;
+  for (int t = threadIdx.x; t < COST_CELLS; t += blockDim.x, ++k) val[t] = v1[k];
+  __syncthreads();
+  // inclination bins with no trained cell: nearest bin that has one
+  for (int t = threadIdx.x; t < COST_CELLS; t += blockDim.x) {
+    const int ia = t / COST_NI, ii = t % COST_NI;
+    float v = val[t];
+    if (!col_have[ii]) {
+      v = 0.0f;
+      for (int d = 1; d < COST_NI; ++d) {
+        if (ii - d >= 0 && col_have[ii - d]) {
+          v = val[ia * COST_NI + ii - d];
+          break;
+        }
+        if (ii + d < COST_NI && col_have[ii + d]) {
+          v = val[ia * COST_NI + ii + d];
+          break;
         }
       }
     }

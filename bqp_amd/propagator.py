@@ -135,7 +135,7 @@ class Propagator:
 
     def cost_model(self) -> dict:
         """The learned cost model (sort_by_cost = 1): per (altitude bin,
-        sin^2 i bin) cell, the evaluations, revolutions and rows summed over
+        cos i bin) cell, the evaluations, revolutions and rows summed over
         the rows that finished ok since the field or configuration was set."""
         cells = nat.COST_ALT_BINS * nat.COST_INC_BINS
         ev = np.zeros(cells, dtype=np.int64)

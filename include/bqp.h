@@ -132,10 +132,10 @@ typedef struct bqp_launch {
 
 /* Cells of the learned cost model: BQP_COST_ALT_BINS log-spaced bins of the
  * altitude of the semi-major axis above the field radius, 100 km to
- * 100,000 km, times BQP_COST_INC_BINS equal bins of sin^2(inclination);
- * cell = alt_bin * BQP_COST_INC_BINS + inc_bin. */
+ * 100,000 km, times BQP_COST_INC_BINS equal bins of cos(inclination) over
+ * [-1, 1]; cell = alt_bin * BQP_COST_INC_BINS + inc_bin. */
 #define BQP_COST_ALT_BINS 96
-#define BQP_COST_INC_BINS 4
+#define BQP_COST_INC_BINS 16
 
 int32_t bqp_default_config(bqp_config* out);
 int32_t bqp_create(const bqp_config* cfg, bqp_ctx** out);

@@ -2,7 +2,7 @@
 
 Each final-state call adds every row of at least one revolution that
 finished ok to a table of (evaluations, revolutions, rows) over cells of
-(altitude of the semi-major axis, sin^2 i); the next call orders rows by revolutions x the learned
+(altitude of the semi-major axis, cos i); the next call orders rows by revolutions x the learned
 evaluations per revolution.  The order never changes results (bit identity
 across orders is in test_gpu_propagate); these tests check that the table
 holds exactly the sums the rows' own counts give, binned as documented in
@@ -29,8 +29,8 @@ def host_cells(y0, span, mu, radius):
     alt = np.maximum(a - radius, 1e5)
     ia = np.clip(np.floor(np.log10(alt / 1e5) * (nat.COST_ALT_BINS / 3.0)).astype(int), 0, nat.COST_ALT_BINS - 1)
     h = np.cross(y0[:, :3], y0[:, 3:])
-    s2 = (h[:, 0] ** 2 + h[:, 1] ** 2) / (h**2).sum(axis=1)
-    ii = np.clip(np.floor(s2 * nat.COST_INC_BINS).astype(int), 0, nat.COST_INC_BINS - 1)
+    ci = h[:, 2] / np.linalg.norm(h, axis=1)
+    ii = np.clip(np.floor(0.5 * (ci + 1.0) * nat.COST_INC_BINS).astype(int), 0, nat.COST_INC_BINS - 1)
     return ia, ii, rev
 
 
