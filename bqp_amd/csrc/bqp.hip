@@ -255,6 +255,10 @@ int default_g(const bqp_ctx* c, int64_t n) {
   return 16;
 }
 
+// First queue position: the first 64 * blocks positions are dealt to the
+// blocks by index (one group each).
+unsigned queue_start(int64_t n, int blocks) { return (unsigned)std::min<int64_t>(n, 64LL * blocks); }
+
 // The learned cost model when the hand-out order uses it, else nullptr.
 CostTable* cost_table(bqp_ctx* c) {
   return c->launch.sort_by_cost == 1 ? (CostTable*)c->d_cost : nullptr;
@@ -296,7 +300,8 @@ void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, prop_fn ker
     hipLaunchKernelGGL(sort_scatter_kernel, dim3(nb), dim3(256), 0, c->stream, bucket, (int)n, hist, order);
     q.order = order;
   }
-  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
+  q.deal_cus = c->cus;
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue, queue_start(n, blocks));
   if (p.sur_totals) hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, p.sur_totals, 5);
   if (events) (void)hipEventRecord(c->ev0, c->stream);
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * G), 0, c->stream, q);
@@ -392,14 +397,16 @@ int run_surrogate_partitioned(bqp_ctx* c, PropParams p, int64_t n, int kprec, in
     q.order = exact_rows;
     q.n = cnt[0];
     q.sur_totals = nullptr;
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
+    q.deal_cus = c->cus;
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue, queue_start(cnt[0], be));
     hipLaunchKernelGGL(exact_kern, dim3(be), dim3(64 * G), 0, c->stream, q);
   }
   if (cnt[1] > 0) {
     PropParams q = p;
     q.order = sur_rows;
     q.n = cnt[1];
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue);
+    q.deal_cus = c->cus;
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue, queue_start(cnt[1], bs));
     hipLaunchKernelGGL(sur_kern, dim3(bs), dim3(64 * G), 0, c->stream, q);
   }
   HIP_TRY(c, hipEventRecord(c->ev1, c->stream));

@@ -154,8 +154,9 @@ __global__ void __launch_bounds__(1024) partition_kernel(const int* __restrict__
   }
 }
 
-// Queue reset without hipMemsetAsync (graph-capture safe).
-__global__ void reset_counter_kernel(unsigned int* q) { q[0] = 0u; }
+// Queue reset without hipMemsetAsync (graph-capture safe): the queue starts
+// after the round of groups dealt by block index.
+__global__ void reset_counter_kernel(unsigned int* q, unsigned int start) { q[0] = start; }
 
 // Longest-first hand-out order (a counting sort made of plain kernels, so it
 // can be captured in a HIP graph).  The key estimates a row's evaluations as

@@ -39,6 +39,21 @@ namespace bqp {
 
 static __constant__ bqp_tableau c_tab[3] = {BQP_DOPRI5_TABLEAU, BQP_DOP853_TABLEAU, BQP_RK87_TABLEAU};
 
+// Hand-out position (in groups) of block b's first group.  Every lane starts
+// free, so the first round of groups is dealt by block index instead of
+// through the queue: blocks b, b + C, b + 2C, ... (C = deal_cus) are the ones
+// the dispatcher places on one CU when the grid fills the device, and a
+// back-and-forth deal over the longest-first order (round j takes positions
+// jC .. jC + C - 1, in reverse on odd rounds) gives each CU groups of ranks
+// c, 2C-1-c, 2C+c, 4C-1-c, ..., so the CUs carry nearly equal work.  Rounds
+// not complete keep block order.  A permutation of [0, nb) in every case.
+__device__ __forceinline__ unsigned first_group(unsigned b, unsigned nb, unsigned C) {
+  if (C == 0u || nb <= C) return b;
+  const unsigned j = b / C, c = b - j * C;
+  if (j < nb / C && (j & 1u)) return j * C + (C - 1u - c);
+  return b;
+}
+
 // Second-order rearrangement of each tableau, computed on the host
 // (make_second_order) and copied to the device when a context is created.
 struct SecondOrderTableau {
@@ -124,8 +139,9 @@ struct PropParams {
   int32_t* steps;
   int32_t* evals;
   int32_t* rejects;
-  unsigned int* queue;
+  unsigned int* queue;   // next queue position; starts after the first round
   int n;
+  int deal_cus;          // CUs the first round of groups is dealt over (0: block order)
   int m_eval;
   double t0c;
   FieldView field;
@@ -542,13 +558,19 @@ propagate_kernel(const PropParams p) {
     }
   };
 
+  bool first_fill = true;
   while (true) {
     // ---------------------------------------------------------- refill
     const unsigned long long need = wave_ballot(!active);
     if (need != 0ull && !exhausted) {
       const int cnt = __popcll(need);
       unsigned int base;
-      if (G == 1) {
+      const bool dealt = first_fill;
+      if (first_fill) {
+        // every lane is free: the block's first group is dealt, not queued
+        base = 64u * first_group(blockIdx.x, gridDim.x, (unsigned)p.deal_cus);
+        first_fill = false;
+      } else if (G == 1) {
         unsigned int b = 0;
         if (lane == __ffsll((long long)need) - 1) b = atomicAdd(p.queue, (unsigned)cnt);
         base = __shfl(b, __ffsll((long long)need) - 1);
@@ -558,7 +580,10 @@ propagate_kernel(const PropParams p) {
         base = lds_base;
         __syncthreads();
       }
-      if (base + (unsigned)cnt >= (unsigned)p.n) exhausted = true;
+      if (dealt)  // the queue starts after the dealt round
+        exhausted = 64ull * gridDim.x >= (unsigned long long)p.n;
+      else if (base + (unsigned)cnt >= (unsigned)p.n)
+        exhausted = true;
       if (!active) {
         const unsigned pos = base + (unsigned)lane_rank(need);
         if (pos < (unsigned)p.n) {
