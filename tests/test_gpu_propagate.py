@@ -302,3 +302,29 @@ def test_graph_replay_matches_direct_launch():
             assert np.array_equal(out.numpy(), ref), rep
             assert np.array_equal(s.numpy(), sref)
             assert info["rows_ok"] == n
+
+
+def test_dealt_first_round_covers_every_row_once():
+    """The first round of groups is dealt to the blocks by index (a
+    back-and-forth deal over the hand-out order) and the queue starts after
+    it.  At 70,000 rows and G = 4 the full grid deals four rounds over the
+    CUs and queues the remaining 70 groups; grids of 700 and 3 blocks deal
+    two rounds plus a partial one, and none.  Every row must run exactly once
+    (status ok, finite state, its own evaluation count), and for one G the
+    results must be bit-identical whatever the grid."""
+    announce("dealt first round: 70,000 rows x 300 s, N=4, G=4, grids full / 700 / 3")
+    w = inputs.workload("w1", n=70000)
+    field = bqp_amd.kaula(4)
+    out = {}
+    with bqp_amd.Propagator(field, method="rk87", rtol=1e-10) as p:
+        for blocks in (0, 700, 3):
+            p.set_launch(waves_per_group=4, blocks=blocks)
+            y, st, info = p.propagate_batch_final(w["y0"], 0.0, np.full(70000, 300.0))
+            out[blocks] = (y, st, info["evals_per_row"].copy())
+    y0, s0, e0 = out[0]
+    assert np.all(s0 == 0) and np.all(np.isfinite(y0)) and np.all(e0 > 0)
+    for blocks in (700, 3):
+        y, st, ev = out[blocks]
+        assert np.array_equal(st, s0)
+        assert np.array_equal(ev, e0)
+        assert np.array_equal(y, y0)
