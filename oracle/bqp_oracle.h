@@ -12,9 +12,11 @@
  *       integrate_internal, PI controller)
  *   dust_transfer/crates/lightyear_odeint_rs/src/odesolve/lightyear_compat.rs
  *   dust_transfer/crates/lightyear_odeint_rs/src/session.rs (ragged batch rows)
- * with the deviations the propagator's north star asks for: fully normalised
- * coefficients, SI units, Cowell (not Encke) state, a uniform z-rotation for the
- * body frame, and the Vallado piecewise exponential atmosphere.
+ * with these deviations from the reference: fully normalised coefficients, SI
+ * units, a Cowell (inertial) state by default with the reference's Encke
+ * formulation as an option, a uniform z-rotation for the body frame, and the
+ * Vallado piecewise exponential atmosphere besides the reference's single-scale
+ * one.
  *
  * Pinning status of each function (what checks it):
  *   orc_gravity_bf (normalised)   Orekit d/o-5 vectors, mpmath brute force

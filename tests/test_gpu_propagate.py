@@ -1,7 +1,7 @@
 """GPU propagation against the CPU oracle on the same seeded inputs.
 
 Bars: final positions within 1 cm of the oracle at equal tolerances (the
-north-star parity bar; observed differences are far smaller), bit-identical
+required parity bar; observed differences are far smaller), bit-identical
 results across launch shapes, NaN-filled failed rows with their status."""
 import numpy as np
 import pytest
