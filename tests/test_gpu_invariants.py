@@ -48,3 +48,29 @@ def test_full_batch_jacobi_integral_and_oracle_sample(orc):
     yo, so, _, _ = orc.propagate_batch(model, integ, y0[idx], 0.0, DAY)
     assert np.all(so == 0)
     assert pos_err(yf[idx], yo).max() < 1e-2
+
+
+def test_full_batch_drag_lowers_every_jacobi_integral():
+    """With drag, dJ/dt = v_rel . a_drag = -(1/2)(Cd A/m) rho |v_rel|^3 <= 0:
+    J must fall for every row, and by more for the lower, draggier orbits."""
+    announce("65,536 LEO x 1 day, N=70, Vallado drag: every Jacobi integral falls")
+    n = 65536
+    w = inputs.workload("w1", n=n)
+    cbar, sbar = inputs.kaula_field(70)
+    field = bqp_amd.kaula(70)
+    with bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado", omega=OMEGA, theta0=0.0) as p:
+        yf, st, _ = p.propagate_batch_final(w["y0"], 0.0, np.full(n, DAY), w["bc"])
+    assert np.all(st == 0)
+    drop = jacobi((cbar, sbar), 70, w["y0"], 0.0) - jacobi((cbar, sbar), 70, yf, OMEGA * DAY)
+    announce(f"J drop: min {drop.min():.3e}, median {np.median(drop):.3e}, max {drop.max():.3e} m^2/s^2")
+    assert np.all(drop > 0.0)
+    # per unit ballistic coefficient, the lowest tenth of the orbits lose more than the highest tenth
+    a = -field.mu / (2.0 * two_body_energy(w["y0"], field.mu))
+    k = drop / w["bc"]
+    lo, hi = a < np.quantile(a, 0.1), a > np.quantile(a, 0.9)
+    assert np.median(k[lo]) > 10.0 * np.median(k[hi])
+
+
+def two_body_energy(y, mu):
+    """Two-body specific energy (for ranking orbits by size)."""
+    return 0.5 * (y[:, 3:] ** 2).sum(axis=1) - mu / np.linalg.norm(y[:, :3], axis=1)
