@@ -7,6 +7,9 @@ Workload (one "step" = one pass of the hot path over the whole batch):
   Kaula field, Vallado piecewise exponential drag with Cd*A/m U(0.005,0.02).
 Inputs are resident in HBM before the timed region; the timed region is the
 device call (cost-sort + persistent propagation kernel + row statistics).
+The untimed warm-up calls also teach the context's cost model (evaluations
+per revolution by altitude and inclination) that orders the rows of the
+timed calls; a call made before any warm-up orders them by revolutions.
 
 `value` is the plain FP64 form.  The split-precision form, the learned
 surrogate (trained on the device first, untimed; BF16 network, exact
