@@ -11,17 +11,22 @@ The untimed warm-up calls also teach the context's cost model (evaluations
 per revolution by altitude and inclination) that orders the rows of the
 timed calls; a call made before any warm-up orders them by revolutions.
 
-`value` is the plain FP64 form.  The split-precision form, the learned
-surrogate (trained on the device first, untimed; BF16 network, exact
-fallback on a failed sampled check) and the Encke formulation (the
-reference's own: deviation from a two-body baseline rebased every 5,400 s,
-absolute error control) run on the same inputs after it; their rates and
-final-position differences from FP64 are reported under "forms".  The CPU baseline is the project's FP64 C
-restatement (oracle/) on a bounded sample of the same workload.
+`value` is the plain FP64 form: W untimed warm-up passes, then K timed
+passes.  Its JSON line (with roofline and cpu_baseline) is printed as soon as
+it is measured.  The split-precision form, the learned surrogate (trained on
+the device first, untimed; BF16 network, exact fallback on a failed sampled
+check) and the Encke formulation (the reference's own: deviation from a
+two-body baseline rebased every 5,400 s, absolute error control) then run on
+the same inputs, one timed pass each after a one-hour warm-up pass, as long
+as their projected time fits --budget-s; a second line repeats the headline
+with their rates and final-position differences from FP64 under "forms".
+The CPU baseline is the project's FP64 C restatement (oracle/) on a bounded
+sample of the same workload, run before any GPU work.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N]
        [--days D] [--forms fp64,split,surrogate,encke] [--no-cpu]
-       [--workload w1|w2|w3]   (w2: ragged batch; w3: surrogate study)
+       [--budget-s S] [--workload w1|w2|w3|w4]
+       (w2: ragged batch; w3: surrogate study; w4: NSGA-II population)
 """
 from __future__ import annotations
 
@@ -57,25 +62,36 @@ def stage_flops(method: str) -> float:
     return float(f + 40)
 
 
-def cpu_baseline(w, field, rows: int, days: float, threads: int) -> dict:
+def cpu_baseline(w, field, threads: int, target_s: float, days: float = 1.0) -> dict:
+    """The oracle (FP64 C restatement, OpenMP over rows) on the first rows of
+    the same workload for `days` each.  A pilot of 2 rows per thread sizes
+    the timed sample to about `target_s` seconds; the rate is the timed
+    sample's orbit-days over its wall time."""
     from oracle import oracle as orc
 
     f = orc.Field(field.cbar, field.sbar, field.nmax, field.mu, field.radius)
     model = orc.make_model(f, drag="vallado")
     integ = orc.make_integ(method="rk87", rtol=1e-12, canonical=(field.radius, field.mu))
-    y0 = w["y0"][:rows]
-    bc = w["bc"][:rows]
-    t0 = time.perf_counter()
-    _, st, _, _ = orc.propagate_batch(model, integ, y0, 0.0, days * 86400.0, bc, threads=threads)
-    wall = time.perf_counter() - t0
-    ok = int(np.sum(st == 0))
+    n_all = w["y0"].shape[0]
+
+    def timed(rows):
+        t0 = time.perf_counter()
+        _, st, _, _ = orc.propagate_batch(model, integ, w["y0"][:rows], 0.0, days * 86400.0, w["bc"][:rows],
+                                          threads=threads)
+        return time.perf_counter() - t0, int(np.sum(st == 0))
+
+    pilot = min(n_all, 2 * threads)
+    pw, _ = timed(pilot)
+    rows = int(min(n_all, max(pilot, round(pilot * target_s / max(pw, 1e-3) / threads) * threads)))
+    wall, ok = timed(rows)
     return {
         "value": rows * days / wall,
         "unit": "orbit-days/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {rows} orbits of the same workload for {days:g} day(s) each "
-                  f"({ok}/{rows} ok), FP64 C restatement, OpenMP dynamic schedule, wall {wall:.1f} s",
+        "sample": f"first {rows} orbits of the same workload for {days:g} day(s) each ({ok}/{rows} ok), FP64 C "
+                  f"restatement of the same algorithm (oracle/), OpenMP dynamic schedule over rows, wall "
+                  f"{wall:.1f} s (sized by a {pilot}-row pilot of {pw:.1f} s)",
     }
 
 
@@ -408,7 +424,100 @@ def cpu_baseline_ragged(w, field, threads: int) -> dict:
                       + "; ".join(parts)}
 
 
+def cpu_threads() -> int:
+    try:
+        return min(16, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return min(16, os.cpu_count() or 1)
+
+
+def run_form(form, args, field, bufs, n, warm_passes):
+    """One form of the headline workload: `warm_passes` untimed passes (a
+    one-hour pass when warm_passes is 0 and --warmup is set), then
+    args.steps or 1 timed passes over the whole batch, each bracketed by a
+    device synchronisation.  Returns (result record, final states)."""
+    import bqp_amd
+    from bqp_amd.device import DeviceArray, synchronize
+
+    y0, t0, tf, tf_short, bc = bufs
+    if form == "encke":  # the reference's formulation: deviation from a rebased two-body baseline
+        prop = bqp_amd.Propagator(field, method="rk87", control="absolute", eps=args.encke_eps,
+                                  drag="vallado", formulation="encke")
+    else:
+        prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado",
+                                  precision="surrogate" if form == "surrogate" else form)
+    if args.launch:
+        prop.set_launch(**json.loads(args.launch))
+    sur = sur_report = None
+    if form == "surrogate":
+        sur, _, _, sur_report = train_surrogate(prop, args.sur_rows, args.sur_steps)
+        sur.pack("bf16")
+        sur.set_check(args.sur_check, 1e-9)
+    frhs = prop.flops_per_rhs
+    fstage = stage_flops("rk87")
+    yf = DeviceArray((n, 6))
+    status = DeviceArray((n,), np.int32)
+    steps_t = DeviceArray((n,), np.int32)
+    evals_t = DeviceArray((n,), np.int32)
+    for i in range(warm_passes):
+        prop.propagate_batch_final_dev(y0, t0, tf, bc, yf, status, steps_t, evals_t)
+        print(f"[bench] {form}: warm-up pass {i + 1}/{warm_passes} done", file=sys.stderr, flush=True)
+    if warm_passes == 0 and args.warmup:
+        # a one-hour pass brings clocks and caches up without a full extra pass
+        prop.propagate_batch_final_dev(y0, t0, tf_short, bc, yf, status, steps_t, evals_t)
+    timed = args.steps if warm_passes else 1
+    synchronize()
+    wall = kern_ms = flops = 0.0
+    for i in range(timed):
+        synchronize()
+        ts = time.perf_counter()
+        _, _, st = prop.propagate_batch_final_dev(y0, t0, tf, bc, yf, status, steps_t, evals_t)
+        synchronize()
+        wall += time.perf_counter() - ts
+        kern_ms += st["kernel_ms"]
+        flops += (st["flops"] if form == "surrogate" else st["evals"] * frhs) + \
+            (st["steps"] + st["rejects"]) * fstage
+        print(f"[bench] {form}: timed pass {i + 1}/{timed}: {st['kernel_ms']:.1f} ms kernel", file=sys.stderr,
+              flush=True)
+    ok = int(np.sum(status.numpy() == 0))
+    ev_rows = evals_t.numpy()
+    ms = wall / timed * 1e3
+    kms = kern_ms / timed
+    achieved = flops / timed / (kms * 1e-3) / 1e12
+    rec = {
+        "value": n * args.days / (ms * 1e-3),
+        "timed_passes": timed,
+        "ms_per_step": ms,
+        "kernel_ms": kms,
+        "rows_ok": ok,
+        "rhs_evals": int(st["evals"]),
+        "steps": int(st["steps"]),
+        "rejects": int(st["rejects"]),
+        "flops_per_rhs": frhs,
+        "achieved_tflops": achieved,
+        "launch": getattr(prop, "launch", None),
+        # per-row work spread: every lane holds one row, so the rows with
+        # the most evaluations set the launch's length
+        "evals_per_row": {"min": int(ev_rows.min()), "mean": float(ev_rows.mean()), "max": int(ev_rows.max())},
+    }
+    if form == "surrogate":
+        rec["surrogate_stats"] = sur.last_stats()
+        rec["training"] = sur_report
+    final = yf.numpy()
+    prop.close()
+    print(f"[bench] {form}: {rec['value']:.1f} orbit-days/s, {ms:.1f} ms/step, kernel {kms:.1f} ms, "
+          f"{achieved:.2f} TFLOP/s, ok {ok}/{n}", file=sys.stderr, flush=True)
+    return rec, final
+
+
+# Cost of one timed pass of each later form, as a multiple of the FP64
+# pass, plus its fixed set-up seconds (surrogate: device training), used to
+# keep the whole run inside --budget-s.
+FORM_COST = {"split": (2.3, 2.0), "surrogate": (1.15, 20.0), "encke": (1.3, 2.0), "fp64": (1.1, 2.0)}
+
+
 def main():
+    t_start = time.perf_counter()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
@@ -416,13 +525,16 @@ def main():
     ap.add_argument("--rows", type=int, default=65536)
     ap.add_argument("--days", type=float, default=7.0)
     ap.add_argument("--forms", default="fp64,split,surrogate,encke")
+    ap.add_argument("--budget-s", type=float, default=450.0,
+                    help="wall-clock budget of the whole run: forms after the headline are skipped when "
+                         "their projected time would cross it")
     ap.add_argument("--encke-eps", type=float, default=1e-6,
                     help="encke form: absolute error bound on the deviation (m, m/s) per step")
     ap.add_argument("--sur-rows", type=int, default=10_000_000, help="surrogate training samples")
     ap.add_argument("--sur-steps", type=int, default=3000, help="surrogate Adam steps (batch 8192)")
     ap.add_argument("--sur-check", type=int, default=100, help="surrogate: check every k-th evaluation")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-rows", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU time of the baseline sample")
     ap.add_argument("--launch", default="", help='JSON launch shape, e.g. {"waves_per_group": 2}')
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="after the run, one device-to-device copy of this many MiB (a known-size kernel for "
@@ -439,6 +551,9 @@ def main():
               f"(--gpus {args.gpus} is not supported; trajectories would shard by cost-weighted "
               f"index ranges, one process per GPU).", file=sys.stderr)
         sys.exit(2)
+    if args.steps < 1 or args.warmup < 0:
+        print("--steps must be >= 1 and --warmup >= 0", file=sys.stderr)
+        sys.exit(2)
 
     if args.workload == "w2":
         print(json.dumps(run_ragged(args)))
@@ -452,129 +567,32 @@ def main():
 
     import bqp_amd
     from bqp_amd import inputs
-    from bqp_amd.device import DeviceArray, synchronize
+    from bqp_amd.device import DeviceArray
 
     n = args.rows
     w = inputs.workload("w1", n=n)
     field = bqp_amd.kaula(70)
     tf_val = args.days * 86400.0
-    # inputs resident in HBM before any timed region
-    y0 = DeviceArray.from_numpy(w["y0"])
-    t0 = DeviceArray.from_numpy(np.zeros(n))
-    tf = DeviceArray.from_numpy(np.full(n, tf_val))
-    tf_short = DeviceArray.from_numpy(np.full(n, min(tf_val, 3600.0)))
-    bc = DeviceArray.from_numpy(w["bc"])
-    orbit_days = n * args.days
-    fstage = stage_flops("rk87")
-
     forms = [f for f in args.forms.split(",") if f]
-    results = {}
-    finals = {}
-    sur_report = None
-    for form in forms:
-        if form == "encke":  # the reference's formulation: deviation from a rebased two-body baseline
-            prop = bqp_amd.Propagator(field, method="rk87", control="absolute", eps=args.encke_eps,
-                                      drag="vallado", formulation="encke")
-        else:
-            prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado",
-                                      precision="surrogate" if form == "surrogate" else form)
-        if args.launch:
-            prop.set_launch(**json.loads(args.launch))
-        if form == "surrogate":
-            sur, _, _, sur_report = train_surrogate(prop, args.sur_rows, args.sur_steps)
-            sur.pack("bf16")
-            sur.set_check(args.sur_check, 1e-9)
-        frhs = prop.flops_per_rhs
-        yf = DeviceArray((n, 6))
-        status = DeviceArray((n,), np.int32)
-        steps_t = DeviceArray((n,), np.int32)
-        evals_t = DeviceArray((n,), np.int32)
-        if form == forms[0]:
-            for _ in range(args.warmup):
-                prop.propagate_batch_final_dev(y0, t0, tf, bc, yf, status, steps_t, evals_t)
-        elif args.warmup:
-            # later forms: a one-hour launch brings clocks and caches up without a full extra pass
-            prop.propagate_batch_final_dev(y0, t0, tf_short, bc, yf, status, steps_t, evals_t)
-        synchronize()
-        wall = 0.0
-        kern_ms = 0.0
-        flops = 0.0
-        for _ in range(args.steps):
-            synchronize()
-            ts = time.perf_counter()
-            _, _, st = prop.propagate_batch_final_dev(y0, t0, tf, bc, yf, status, steps_t, evals_t)
-            synchronize()
-            wall += time.perf_counter() - ts
-            kern_ms += st["kernel_ms"]
-            flops += (st["flops"] if form == "surrogate" else st["evals"] * frhs) + \
-                (st["steps"] + st["rejects"]) * fstage
-        ok = int(np.sum(status.numpy() == 0))
-        ev_rows = evals_t.numpy()
-        ms = wall / args.steps * 1e3
-        kms = kern_ms / args.steps
-        achieved = flops / args.steps / (kms * 1e-3) / 1e12
-        results[form] = {
-            "value": orbit_days / (ms * 1e-3),
-            "ms_per_step": ms,
-            "kernel_ms": kms,
-            "rows_ok": ok,
-            "rhs_evals": int(st["evals"]),
-            "steps": int(st["steps"]),
-            "rejects": int(st["rejects"]),
-            "flops_per_rhs": frhs,
-            "achieved_tflops": achieved,
-            "launch": getattr(prop, "launch", None),
-            # per-row work spread: every lane holds one row, so the rows with
-            # the most evaluations set the launch's length
-            "evals_per_row": {"min": int(ev_rows.min()), "mean": float(ev_rows.mean()),
-                              "max": int(ev_rows.max())},
-        }
-        if form == "surrogate":
-            results[form]["surrogate_stats"] = sur.last_stats()
-            results[form]["training"] = sur_report
-        finals[form] = yf.numpy()
-        prop.close()
-        print(f"[bench] {form}: {results[form]['value']:.1f} orbit-days/s, {ms:.1f} ms/step, "
-              f"kernel {kms:.1f} ms, {achieved:.2f} TFLOP/s, ok {ok}/{n}", file=sys.stderr, flush=True)
+    if not forms:
+        print("--forms names no form", file=sys.stderr)
+        sys.exit(2)
 
-    base = results.get("fp64") or next(iter(results.values()))
-    forms_out = {}
-    if "fp64" in finals:
-        for form in forms:
-            if form == "fp64":
-                continue
-            d = np.linalg.norm(finals[form][:, :3] - finals["fp64"][:, :3], axis=1)
-            d = np.nan_to_num(d, nan=np.inf)
-            forms_out[form] = {
-                "value": results[form]["value"],
-                "speedup_vs_fp64": results[form]["value"] / results["fp64"]["value"],
-                "max_pos_err_m_vs_fp64": float(d.max()),
-                "median_pos_err_m_vs_fp64": float(np.median(d)),
-                "achieved_tflops": results[form]["achieved_tflops"],
-                "kernel_ms": results[form]["kernel_ms"],
-            }
-            if form == "encke":
-                forms_out[form].update({
-                    "formulation": "Encke deviation on a two-body baseline, Battin correction, rebased every "
-                                   "5,400 s (fresh controller per segment), absolute error control",
-                    "eps": args.encke_eps, "rhs_evals": results[form]["rhs_evals"],
-                    "steps": results[form]["steps"]})
-            if form == "surrogate":
-                forms_out[form].update({
-                    "network": "16 features -> 4 x 256 SiLU -> 3, BF16 MFMA, FP32 accumulate, residual above deg 4",
-                    "check_every": args.sur_check, "threshold_m_s2": 1e-9,
-                    "stats": results[form]["surrogate_stats"], "training": results[form]["training"]})
-
+    # the CPU baseline runs first, before any GPU work is timed
     cpu = None
     if not args.no_cpu:
-        try:
-            threads = min(16, len(os.sched_getaffinity(0)))
-        except AttributeError:
-            threads = min(16, os.cpu_count() or 1)
-        rows = args.cpu_rows or 24 * threads
-        cpu = cpu_baseline(w, field, rows, 1.0, threads)
-        print(f"[bench] cpu baseline {cpu['value']:.3f} orbit-days/s on {threads} threads", file=sys.stderr)
+        cpu = cpu_baseline(w, field, cpu_threads(), args.cpu_seconds)
+        print(f"[bench] cpu baseline {cpu['value']:.3f} orbit-days/s on {cpu['cores']} threads", file=sys.stderr,
+              flush=True)
 
+    # inputs resident in HBM before any timed region
+    bufs = (DeviceArray.from_numpy(w["y0"]), DeviceArray.from_numpy(np.zeros(n)),
+            DeviceArray.from_numpy(np.full(n, tf_val)), DeviceArray.from_numpy(np.full(n, min(tf_val, 3600.0))),
+            DeviceArray.from_numpy(w["bc"]))
+
+    # the headline form: --warmup full passes, then --steps timed passes
+    head = forms[0]
+    base, head_final = run_form(head, args, field, bufs, n, args.warmup)
     line = {
         "metric": METRIC,
         "value": base["value"],
@@ -586,7 +604,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if head in ("fp64", "encke") else ("f64+f32" if head == "split" else "f64+bf16"),
         "data": "synthetic: seeded LEO elements (PCG64 seed 0) and seeded Kaula degree-70 field",
         "config": {
             "workload": f"{n} LEO orbits x {args.days:g} days, RK8(7)13M rtol 1e-12, degree/order 70, "
@@ -597,7 +615,7 @@ def main():
             "method": "RK8(7)13M",
             "rtol": 1e-12,
             "drag": "vallado",
-            "form": "fp64",
+            "form": head,
         },
         "roofline": {
             "bound": "mfma",
@@ -606,7 +624,7 @@ def main():
             "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": base["achieved_tflops"] / FP64_PEAK_TFLOPS,
-            "traffic": measured_traffic() if (n == 65536 and args.days == 7.0) else None,
+            "traffic": None,
             "kernel": "bqp::propagate_kernel<RK87, FP64, G>",
             "kernel_ms": base["kernel_ms"],
             "flops_per_rhs": base["flops_per_rhs"],
@@ -614,9 +632,53 @@ def main():
             "evals_per_row": base["evals_per_row"],
         },
         "cpu_baseline": cpu,
-        "forms": forms_out,
     }
-    print(json.dumps(line))
+    traffic = committed_traffic() if (n == 65536 and args.days == 7.0 and head == "fp64") else None
+    if traffic is not None:
+        # HBM bytes per launch cannot be counted inside this process (the
+        # counters need rocprofv3 --pmc); the figure is the committed one
+        line["roofline"]["traffic"] = traffic["bytes"]
+        line["roofline"]["traffic_source"] = traffic["source"]
+    print(json.dumps(line), flush=True)
+
+    # the other forms: one timed pass each, inside the wall-clock budget
+    forms_out, skipped = {}, {}
+    for form in forms[1:]:
+        factor, setup = FORM_COST.get(form, (2.5, 5.0))
+        projected = setup + (factor * base["ms_per_step"] + 3.6e6 / (args.days * 86400.0) * base["ms_per_step"]) * 1e-3
+        elapsed = time.perf_counter() - t_start
+        if elapsed + projected > args.budget_s:
+            skipped[form] = f"skipped: {elapsed:.0f} s elapsed + {projected:.0f} s projected > {args.budget_s:.0f} s"
+            print(f"[bench] {form}: {skipped[form]}", file=sys.stderr, flush=True)
+            continue
+        rec, final = run_form(form, args, field, bufs, n, 0)
+        d = np.nan_to_num(np.linalg.norm(final[:, :3] - head_final[:, :3], axis=1), nan=np.inf)
+        out = {
+            "value": rec["value"],
+            f"speedup_vs_{head}": rec["value"] / base["value"],
+            f"max_pos_err_m_vs_{head}": float(d.max()),
+            f"median_pos_err_m_vs_{head}": float(np.median(d)),
+            "achieved_tflops": rec["achieved_tflops"],
+            "kernel_ms": rec["kernel_ms"],
+            "timed_passes": rec["timed_passes"],
+            "rows_ok": rec["rows_ok"],
+            "rhs_evals": rec["rhs_evals"],
+        }
+        if form == "encke":
+            out.update({"formulation": "Encke deviation on a two-body baseline, Battin correction, rebased every "
+                                       "5,400 s (fresh controller per segment), absolute error control",
+                        "eps": args.encke_eps, "steps": rec["steps"]})
+        if form == "surrogate":
+            out.update({"network": "16 features -> 4 x 256 SiLU -> 3, BF16 MFMA, FP32 accumulate, residual above "
+                                   "deg 4", "check_every": args.sur_check, "threshold_m_s2": 1e-9,
+                        "stats": rec["surrogate_stats"], "training": rec["training"]})
+        forms_out[form] = out
+    if len(forms) > 1:
+        line["forms"] = forms_out
+        if skipped:
+            line["forms_skipped"] = skipped
+        line["wall_s"] = time.perf_counter() - t_start
+        print(json.dumps(line), flush=True)
     if args.copy_mib:
         copy_calibration(args.copy_mib)
 
@@ -637,14 +699,15 @@ def copy_calibration(mib: int):
     print(f"[bench] calibration copy of {mib} MiB ({a.nbytes} bytes), rc {rc}", file=sys.stderr, flush=True)
 
 
-def measured_traffic():
-    """HBM bytes per launch of the propagation kernel on this workload, from
-    the committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE, scaled
-    by the copy-kernel calibration); None if absent."""
-    path = os.path.join(ROOT, "profiles", "propagate_rk87_deg70_leo65536_7d_traffic.json")
+def committed_traffic():
+    """HBM bytes per launch of the propagation kernel on the headline
+    workload, from the committed rocprofv3 counter summary (FETCH_SIZE +
+    WRITE_SIZE, scaled by a copy kernel's calibration); None if absent."""
+    rel = os.path.join("profiles", "propagate_rk87_deg70_leo65536_7d_traffic.json")
     try:
-        with open(path) as fh:
-            return float(json.load(fh)["hbm_bytes_per_launch"])
+        with open(os.path.join(ROOT, rel)) as fh:
+            return {"bytes": float(json.load(fh)["hbm_bytes_per_launch"]),
+                    "source": f"{rel} (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this workload, not this run)"}
     except (OSError, ValueError, KeyError, TypeError):
         return None
 

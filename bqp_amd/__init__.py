@@ -13,10 +13,13 @@ They run on the GPU through the C ABI (include/bqp.h); there is no CPU path.
 The SI batch API is ``bqp_amd.Propagator``.
 
 Scope differences from the reference binding (the force model of the hot
-path only): ``srp``/``sun``/``moon`` must be False; ``atm_model`` 1 is the
-reference's single-scale-height exponential model, 2 the Vallado piecewise
-table; ``method`` is one of dopri5 (reference compat controller), dop853,
-rk87 (Prince-Dormand RK8(7)13M).  The field is the GOCE DIR-R6 d/o-15 file the
+path only).  The keyword defaults are the reference's (srp, sun and moon on,
+``atm_model=7`` = JB2008, ``method="vern7"``), and those are outside this
+propagator, so a call that relies on them raises ``ValueError``: pass
+``srp=False, sun=False, moon=False``, an ``atm_model`` of 1 (the
+reference's single-scale-height exponential model) or 2 (the Vallado
+piecewise table), and a ``method`` of dopri5 (reference compat controller),
+dop853 or rk87 (Prince-Dormand RK8(7)13M); ``IN_SCOPE`` holds one such set.  The field is the GOCE DIR-R6 d/o-15 file the
 reference bakes in, truncated to ``gravity_order``.  As in the reference, the
 integrated state is the Encke deviation from a two-body baseline rebased
 every 5,400 s under absolute error control (``tol`` km); states in and out
@@ -39,13 +42,17 @@ from .surrogate import Surrogate
 __all__ = [
     "Propagator", "Surrogate", "Field", "BqpError", "kaula", "goce", "load_icgem", "tableau", "inputs", "elements",
     "kep_to_eci", "eci_to_kep", "propagate", "propagate_final", "propagate_batch_final", "gmst",
-    "ROW_STATUS",
+    "ROW_STATUS", "IN_SCOPE",
 ]
 
 MU_KM = 398600.4415  # km^3/s^2, the reference's constant (satpy_core/src/lib.rs:74)
 WGS84_FLATTENING = 1.0 / 298.257223563
 GROUND_ALTITUDE_M = 100e3  # the reference's ground event altitude (lightyear_odeint_rs/src/types.rs:140)
 MAX_GRAVITY_ORDER = 15
+# keyword set of the reference-shaped calls that this propagator covers
+# (gravity + drag, an explicit Runge-Kutta pair); the reference's defaults
+# name SRP, Sun, Moon, JB2008 and Vern7, which raise ValueError here
+IN_SCOPE = {"srp": False, "sun": False, "moon": False, "atm_model": 1, "method": "dop853"}
 
 
 def _six(value, what):
@@ -110,13 +117,15 @@ def _model(gravity_order, drag, srp, sun, moon, atm_model, am_ratio, cd, cr, tol
     if not (math.isfinite(tol) and tol > 0 and math.isfinite(dt_max_s) and dt_max_s > 0):
         raise ValueError("tol and dt_max_s must be finite and > 0")
     if srp or sun or moon:
-        raise ValueError("srp, sun and moon perturbations are outside this propagator; pass srp=False, "
-                         "sun=False, moon=False")
+        raise ValueError("srp, sun and moon perturbations are outside this propagator (it covers gravity and "
+                         "drag); pass srp=False, sun=False, moon=False (bqp_amd.IN_SCOPE holds an in-scope set)")
     if drag and atm_model not in (1, 2):
-        raise ValueError(f"unknown atm_model {atm_model}; use 1 (exponential) or 2 (Vallado piecewise)")
+        raise ValueError(f"atm_model {atm_model} is outside this propagator; use 1 (exponential) or 2 (Vallado "
+                         "piecewise)")
     name = str(method).lower()
     if name not in _METHOD_ALIASES:
-        raise ValueError(f"unknown integrator method {method!r}; use one of dopri5, dop853, rk87")
+        raise ValueError(f"integrator method {method!r} is outside this propagator; use one of dopri5, dop853, "
+                         "rk87")
     meth, control = _METHOD_ALIASES[name]
     field = goce(int(gravity_order))
     # the reference's equations of motion: Encke deviations from a two-body
@@ -133,8 +142,8 @@ def _model(gravity_order, drag, srp, sun, moon, atm_model, am_ratio, cd, cr, tol
     return field, kw, am_ratio * cd
 
 
-def propagate_final(state_eci, epoch_jd, tof_s, *, gravity_order=5, drag=True, srp=False, sun=False, moon=False,
-                    atm_model=1, am_ratio=0.01, cd=2.2, cr=1.3, tol=1e-8, dt_max_s=300.0, method="dop853"):
+def propagate_final(state_eci, epoch_jd, tof_s, *, gravity_order=5, drag=True, srp=True, sun=True, moon=True,
+                    atm_model=7, am_ratio=0.01, cd=2.2, cr=1.3, tol=1e-8, dt_max_s=300.0, method="vern7"):
     """Propagate one ECI state (km, km/s) and return the final state; raises
     ValueError for physically infeasible arcs (ground impact) and RuntimeError
     on integration failure."""
@@ -179,8 +188,8 @@ def _reached_samples(grid, states, status):
     return np.array(grid[:reached]), states[:reached]
 
 
-def propagate(state_eci, epoch_jd, tof_s, times_s=None, *, gravity_order=5, drag=True, srp=False, sun=False,
-              moon=False, atm_model=1, am_ratio=0.01, cd=2.2, cr=1.3, tol=1e-8, dt_max_s=300.0, method="dop853"):
+def propagate(state_eci, epoch_jd, tof_s, times_s=None, *, gravity_order=5, drag=True, srp=True, sun=True,
+              moon=True, atm_model=7, am_ratio=0.01, cd=2.2, cr=1.3, tol=1e-8, dt_max_s=300.0, method="vern7"):
     """Sampled propagation: returns a dict with ``times_s``, ``states_eci``
     (n, 6) km, ``completed``, ``terminal_event``, ``max_steps_exceeded``,
     ``steps``, ``rhs_evals``, ``wall_us``.  The grid always starts at 0 and
@@ -225,9 +234,9 @@ def propagate(state_eci, epoch_jd, tof_s, times_s=None, *, gravity_order=5, drag
     }
 
 
-def propagate_batch_final(states_eci, tof_s, *, epoch_jd=2451545.0, gravity_order=5, drag=True, srp=False,
-                          sun=False, moon=False, atm_model=1, am_ratio=0.01, cd=2.2, cr=1.3, tol=1e-8,
-                          dt_max_s=300.0, method="dop853"):
+def propagate_batch_final(states_eci, tof_s, *, epoch_jd=2451545.0, gravity_order=5, drag=True, srp=True,
+                          sun=True, moon=True, atm_model=7, am_ratio=0.01, cd=2.2, cr=1.3, tol=1e-8,
+                          dt_max_s=300.0, method="vern7"):
     """Batch of ECI states (n, 6) km with per-row tof (n,) s -> (states (n, 6) km, status (n,)).
     Failed rows are NaN-filled; ``status`` codes are in ``ROW_STATUS``."""
     y = np.asarray(states_eci, dtype=np.float64).reshape(-1, 6)

@@ -30,6 +30,7 @@ struct GraphKey {
   bqp_config cfg;
   int G, blocks, sort;
   const void* tab;
+  const void* sort_buf;  // the hand-out tables' buffer (hist, order, bucket), not in p
   bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
 };
 
@@ -53,10 +54,20 @@ int ensure(bqp_ctx* c, void** ptr, size_t* have, size_t need) { return bqp_ensur
 
 }  // namespace
 
+// Drops the captured launch sequence: it holds the addresses of the buffers
+// it was captured with, so it must not be replayed once one of them is freed.
+void bqp_drop_graph(bqp_ctx* c) {
+  if (c->graph_exec) {
+    (void)hipGraphExecDestroy(c->graph_exec);
+    c->graph_exec = nullptr;
+  }
+}
+
 int bqp_ensure(bqp_ctx* c, void** ptr, size_t* have, size_t need) {
   if (*have >= need && *ptr) return BQP_OK;
   if (*ptr) {
     (void)hipStreamSynchronize(c->stream);
+    bqp_drop_graph(c);
     (void)hipFree(*ptr);
     *ptr = nullptr;
     *have = 0;
@@ -553,15 +564,13 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
     key.blocks = blocks;
     key.sort = (sort ? 8 : 0) | c->launch.sort_by_cost;  // the mode decides the cost-model kernels
     key.tab = c->d_tab;
+    key.sort_buf = sort ? c->d_sort : nullptr;
     if (!c->graph_key) {
       c->graph_key = new (std::nothrow) GraphKey();
       if (!c->graph_key) return fail(c, BQP_E_DEVICE, "out of host memory");
     }
     if (!c->graph_exec || !(key == *c->graph_key)) {
-      if (c->graph_exec) {
-        (void)hipGraphExecDestroy(c->graph_exec);
-        c->graph_exec = nullptr;
-      }
+      bqp_drop_graph(c);
       HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       enqueue_propagation(c, p, n, kern, G, blocks, sort, false);
       hipGraph_t graph = nullptr;
@@ -747,6 +756,8 @@ int32_t bqp_set_launch(bqp_ctx* c, const bqp_launch* l) {
   if (l->blocks < 0) return fail(c, BQP_E_INVALID, "blocks must be >= 0");
   if (l->use_graph != 0 && l->use_graph != 1) return fail(c, BQP_E_INVALID, "use_graph must be 0 or 1");
   if (l->sort_by_cost < 0 || l->sort_by_cost > 2) return fail(c, BQP_E_INVALID, "sort_by_cost must be 0, 1 or 2");
+  (void)hipStreamSynchronize(c->stream);
+  bqp_drop_graph(c);
   c->launch = *l;
   return BQP_OK;
 }
@@ -794,6 +805,7 @@ int32_t bqp_load_field(bqp_ctx* c, int32_t nmax, int32_t mmax, const double* cba
                          (float)e.yv, (float)e.yw, (float)e.zv, (float)e.zw};
   }
   (void)hipStreamSynchronize(c->stream);
+  bqp_drop_graph(c);
   bqp_mlp_release(c);  // a surrogate belongs to the field it was set up for
   (void)hipFree(c->d_tab);
   (void)hipFree(c->d_tabf);

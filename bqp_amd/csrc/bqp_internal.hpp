@@ -80,6 +80,7 @@ struct bqp_ctx {
 // helpers shared by the translation units
 int bqp_fail(bqp_ctx* c, int code, const std::string& msg);
 int bqp_ensure(bqp_ctx* c, void** ptr, size_t* have, size_t need);
+void bqp_drop_graph(bqp_ctx* c);
 void bqp_mlp_release(bqp_ctx* c);
 int bqp_mlp_alloc_init(bqp_ctx* c, uint64_t seed);
 

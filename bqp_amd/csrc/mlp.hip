@@ -433,6 +433,7 @@ void bqp_mlp_release(bqp_ctx* c) {
   if (!c || !c->mlp) return;
   bqp_mlp_state* s = c->mlp;
   (void)hipStreamSynchronize(c->stream);
+  bqp_drop_graph(c);
   (void)hipFree(s->d_params);
   (void)hipFree(s->d_grad);
   (void)hipFree(s->d_m);
@@ -650,6 +651,8 @@ int32_t bqp_surrogate_pack(bqp_ctx* c, int32_t format, const float* d_calib, int
     for (int i = 0; i < MLP_WIDTH; ++i) bias[(size_t)l * MLP_WIDTH + i] = P[mlp_b_off(l) + i];
   for (int i = 0; i < MLP_OUT; ++i) bias[4 * MLP_WIDTH + i] = P[mlp_b_off(4) + i];
 
+  (void)hipStreamSynchronize(c->stream);
+  bqp_drop_graph(c);
   (void)hipFree(s->d_packed);
   (void)hipFree(s->d_bias);
   s->d_packed = nullptr;

@@ -36,6 +36,28 @@ def _ip(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_int32))
 
 
+def _check_dev(buf, name, n, tail, dtype, optional=False):
+    """Shape and dtype contract of a device buffer passed to a device-pointer
+    entry point: the kernel indexes it as (n,) + tail elements of `dtype`
+    with no bounds of its own, so a mismatch must be refused on the host.
+    Returns the buffer's row count.  Raises ValueError."""
+    if buf is None:
+        if optional:
+            return n
+        raise ValueError(f"{name} is required")
+    shape = getattr(buf, "shape", None)
+    dt = getattr(buf, "dtype", None)
+    if shape is None or dt is None or not hasattr(buf, "data_ptr"):
+        raise ValueError(f"{name} must be a device buffer with shape, dtype and data_ptr()")
+    shape = tuple(int(s) for s in shape)
+    if len(shape) != 1 + len(tail) or shape[1:] != tuple(tail) or (n is not None and shape[0] != n):
+        want = ("n" if n is None else str(n),) + tuple(str(t) for t in tail)
+        raise ValueError(f"{name} must have shape ({', '.join(want)}), got {shape}")
+    if np.dtype(dt) != np.dtype(dtype):
+        raise ValueError(f"{name} must be {np.dtype(dtype).name}, got {np.dtype(dt).name}")
+    return shape[0]
+
+
 def tableau(method: str) -> dict:
     t = Tableau()
     rc = nat.load().bqp_get_tableau(nat.METHODS[method], C.byref(t))
@@ -205,11 +227,18 @@ class Propagator:
         times, int32 counters).  No host staging.  Returns (yf, status, stats)."""
         from .device import DeviceArray
 
-        n = int(y0.shape[0])
+        n = _check_dev(y0, "y0", None, (6,), np.float64)
+        _check_dev(t0, "t0", n, (), np.float64)
+        _check_dev(tf, "tf", n, (), np.float64)
+        _check_dev(bc, "bc", n, (), np.float64, optional=True)
         if yf is None:
             yf = DeviceArray((n, 6), np.float64)
         if status is None:
             status = DeviceArray((n,), np.int32)
+        _check_dev(yf, "yf", n, (6,), np.float64)
+        _check_dev(status, "status", n, (), np.int32)
+        _check_dev(steps, "steps", n, (), np.int32, optional=True)
+        _check_dev(evals, "evals", n, (), np.int32, optional=True)
         st = nat.Stats()
 
         def ptr(x):

@@ -543,6 +543,17 @@ static int above_ground(const orc_model* md, const double* y) {
   return d - surface >= md->r_min - md->r_atm;
 }
 
+/* Sample times an integration lands on without restarting its controller
+ * (forced steps onto sample times, LY/src/odesolve/solver.rs:1356-1368):
+ * each step is shortened onto the next time in (ta, tb], the inertial state
+ * there is written to out[k], and the time sum's compensation restarts. */
+typedef struct orc_stops {
+  const double* t;  /* increasing sample times */
+  int64_t m;
+  int64_t next;     /* first sample not yet written */
+  double* out;      /* m x 6 */
+} orc_stops;
+
 /* One adaptive integration of y over [ta, tb] with a fresh controller:
  * LY/src/odesolve/solver.rs:1099-1925 integrate_internal with the PI
  * controller macros :445-529 (ORC_CONTROL_SCALED / _ABSOLUTE), or
@@ -553,7 +564,8 @@ static int above_ground(const orc_model* md, const double* y) {
  * return *h_exit holds the step size the controller asked for after its last
  * accepted step (solver.rs:497-498 final_controller_h), 0 if none. */
 static int integrate_segment(rhs_ctx* rc, const orc_integ* cfg, const bqp_tableau* T, double y[6],
-                             double ta, double tb, orc_stats* loc, double h_open, double* h_exit) {
+                             double ta, double tb, orc_stats* loc, double h_open, double* h_exit,
+                             orc_stops* stops) {
   const double span = tb - ta;
   if (h_exit) *h_exit = 0.0;
   if (span == 0.0) return ORC_OK;
@@ -593,7 +605,8 @@ static int integrate_segment(rhs_ctx* rc, const orc_integ* cfg, const bqp_tablea
       loc->status = ORC_MAX_STEPS;
       return loc->status;
     }
-    const double rem = tb - t;
+    const double tgt = (stops && stops->next < stops->m && stops->t[stops->next] < tb) ? stops->t[stops->next] : tb;
+    const double rem = tgt - t;
     double hs = (h < rem) ? h : rem;
     const int lands = (hs == rem);
     if (!compat && hs < h_min && !lands) {
@@ -660,7 +673,7 @@ static int integrate_segment(rhs_ctx* rc, const orc_integ* cfg, const bqp_tablea
     if (en <= thresh || hs <= h_min) {
       double tn;
       if (lands) {
-        tn = tb;
+        tn = tgt;
       } else {
         const double ky = hs - t_comp;
         tn = t + ky;
@@ -709,6 +722,16 @@ static int integrate_segment(rhs_ctx* rc, const orc_integ* cfg, const bqp_tablea
       err_prev = en;
       have_prev = 1;
       just_rej = 0;
+      if (stops && stops->next < stops->m && t == stops->t[stops->next]) {
+        double tot[6];
+        if (total_state(rc, t, y, tot) != ORC_OK) {
+          loc->status = ORC_NON_FINITE;
+          return loc->status;
+        }
+        memcpy(stops->out + 6 * stops->next, tot, sizeof(tot));
+        ++stops->next;
+        t_comp = 0.0;
+      }
     } else {
       ++rejects;
       ++loc->rejects;
@@ -743,8 +766,8 @@ static int integrate_segment(rhs_ctx* rc, const orc_integ* cfg, const bqp_tablea
  * started from the current total state, then rebased.  (The reference's
  * 10 km deviation test only decides how the last segment's result is
  * expressed, :941-951, so the total state is the same.) */
-int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const double y0[6],
-                  double t0, double tf, double yf[6], orc_stats* st) {
+static int propagate_impl(const orc_model* md, const orc_integ* cfg, double bc, const double y0[6],
+                          double t0, double tf, double yf[6], orc_stats* st, orc_stops* stops) {
   orc_stats loc = {0, 0, 0, ORC_OK};
   const bqp_tableau* T = tableau_of(cfg->method);
   double y[6];
@@ -757,12 +780,19 @@ int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const do
     loc.status = ORC_BELOW_MIN_RADIUS;
     goto fail;
   }
+  if (stops) /* samples at the start time are the initial state */
+    while (stops->next < stops->m && stops->t[stops->next] == t0) {
+      memcpy(stops->out + 6 * stops->next, y, sizeof(y));
+      ++stops->next;
+    }
   rhs_ctx rc;
   memset(&rc, 0, sizeof(rc));
   rc.md = md;
   rc.bc = bc;
-  if (md->formulation != ORC_ENCKE) {
-    if (integrate_segment(&rc, cfg, T, y, t0, tf, &loc, 0.0, NULL) != ORC_OK) goto fail;
+  if (tf == t0) {
+    /* nothing to integrate */
+  } else if (md->formulation != ORC_ENCKE) {
+    if (integrate_segment(&rc, cfg, T, y, t0, tf, &loc, 0.0, NULL, stops) != ORC_OK) goto fail;
   } else {
     /* Step-size carry (LY/src/integrator.rs:341-402 hcarry_take/_store): a
      * rebased segment opens with the step size the controller asked for at
@@ -786,7 +816,8 @@ int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const do
         goto fail;
       }
       double h_exit = 0.0;
-      if (integrate_segment(&rc, cfg, T, d, ts, te, &loc, carry_ok ? carried : 0.0, &h_exit) != ORC_OK) goto fail;
+      if (integrate_segment(&rc, cfg, T, d, ts, te, &loc, carry_ok ? carried : 0.0, &h_exit, stops) != ORC_OK)
+        goto fail;
       carried = h_exit;
       if (total_state(&rc, te, d, y) != ORC_OK) {
         loc.status = ORC_NON_FINITE;
@@ -801,8 +832,60 @@ int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const do
   return ORC_OK;
 fail:
   for (int i = 0; i < 6; ++i) yf[i] = NAN;
+  if (stops) /* a failed row's unreached samples are NaN */
+    for (int64_t k = stops->next; k < stops->m; ++k)
+      for (int i = 0; i < 6; ++i) stops->out[6 * k + i] = NAN;
   if (st) *st = loc;
   return loc.status;
+}
+
+int orc_propagate(const orc_model* md, const orc_integ* cfg, double bc, const double y0[6],
+                  double t0, double tf, double yf[6], orc_stats* st) {
+  return propagate_impl(md, cfg, bc, y0, t0, tf, yf, st, NULL);
+}
+
+/* Sampled propagation (LY/src/batch.rs:247-390 integrate_batch_native_into
+ * with t_eval; DT/src/lib.rs:237-353 propagate(times_s=...)): one integration
+ * from t0 to the last sample time whose steps are shortened onto every
+ * sample time, with the controller carried through the samples (Cowell: one
+ * controller for the whole row; Encke: one per 5,400 s segment, samples
+ * inside a segment do not restart it).  out holds m x 6 inertial states. */
+int orc_propagate_sampled(const orc_model* md, const orc_integ* cfg, double bc, const double y0[6], double t0,
+                          const double* t_eval, int64_t m, double* out, orc_stats* st) {
+  double yf[6];
+  orc_stops stops = {t_eval, m, 0, out};
+  int bad = (m <= 0 || !isfinite(t0));
+  for (int64_t k = 0; k < m && !bad; ++k)
+    if (!isfinite(t_eval[k]) || t_eval[k] < t0 || (k > 0 && !(t_eval[k] > t_eval[k - 1]))) bad = 1;
+  if (bad) {
+    for (int64_t k = 0; k < m * 6; ++k) out[k] = NAN;
+    if (st) {
+      orc_stats z = {0, 0, 0, ORC_INVALID_INPUT};
+      *st = z;
+    }
+    return ORC_INVALID_INPUT;
+  }
+  return propagate_impl(md, cfg, bc, y0, t0, t_eval[m - 1], yf, st, &stops);
+}
+
+int orc_propagate_sampled_batch(const orc_model* md, const orc_integ* cfg, const double* y0, double t0,
+                                const double* t_eval, int64_t m, const double* bc, int64_t n, double* out,
+                                int32_t* status, int64_t* steps, int64_t* evals, int threads) {
+#ifdef _OPENMP
+  if (threads <= 0) threads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
+#endif
+  for (int64_t i = 0; i < n; ++i) {
+    orc_stats st;
+    orc_propagate_sampled(md, cfg, bc ? bc[i] : 0.0, y0 + 6 * i, t0, t_eval, m, out + 6 * m * i, &st);
+    if (status) status[i] = st.status;
+    if (steps) steps[i] = st.steps;
+    if (evals) evals[i] = st.evals;
+  }
+  (void)threads;
+  for (int64_t i = 0; i < n && status; ++i)
+    if (status[i] != ORC_OK) return status[i];
+  return ORC_OK;
 }
 
 /* Ragged row-parallel batch (LY/src/session.rs:2325-2436): rows are

@@ -120,6 +120,16 @@ void orc_rhs(const orc_model* m, double bc, double t, const double y[6], double 
 int orc_propagate(const orc_model* m, const orc_integ* cfg, double bc, const double y0[6],
                   double t0, double tf, double yf[6], orc_stats* st);
 
+/* Sampled propagation from t0 to the last of the m increasing sample times
+ * t_eval (>= t0), steps shortened onto every sample (the controller runs
+ * through them); out receives m x 6 inertial states, NaN from the first
+ * unreached sample of a failed row on. */
+int orc_propagate_sampled(const orc_model* m, const orc_integ* cfg, double bc, const double y0[6], double t0,
+                          const double* t_eval, int64_t m_eval, double* out, orc_stats* st);
+int orc_propagate_sampled_batch(const orc_model* m, const orc_integ* cfg, const double* y0, double t0,
+                                const double* t_eval, int64_t m_eval, const double* bc, int64_t n, double* out,
+                                int32_t* status, int64_t* steps, int64_t* evals, int threads);
+
 /* Ragged batch: row i runs from t0[i] to tf[i] with ballistic coefficient
  * bc[i] (Cd*A/m, m^2/kg; NULL = 0).  Failed rows are NaN-filled. */
 int orc_propagate_batch(const orc_model* m, const orc_integ* cfg, const double* y0,

@@ -9,6 +9,7 @@ per-function citations and pinning status.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -84,25 +85,46 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = C.CDLL(_LIB_PATH)
-        L.orc_field_create.restype = C.c_void_p
-        L.orc_field_create.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int]
-        L.orc_field_destroy.argtypes = [C.c_void_p]
-        L.orc_gravity_bf.argtypes = [C.c_void_p, _dp, _dp]
-        L.orc_gravity_bf_batch.argtypes = [C.c_void_p, _dp, C.c_int64, _dp, C.c_int]
-        L.orc_density.restype = C.c_double
-        L.orc_density.argtypes = [C.c_int, C.c_double]
-        L.orc_rhs.argtypes = [C.POINTER(Model), C.c_double, C.c_double, _dp, _dp]
-        L.orc_propagate.argtypes = [C.POINTER(Model), C.POINTER(Integ), C.c_double, _dp,
-                                    C.c_double, C.c_double, _dp, C.POINTER(Stats)]
-        L.orc_propagate_batch.argtypes = [C.POINTER(Model), C.POINTER(Integ), _dp, _dp, _dp, _dp,
-                                          C.c_int64, _dp, _i32p, _i64p, _i64p, C.c_int]
-        L.orc_kep2eci.argtypes = [C.c_double, _dp, _dp]
-        L.orc_tableau.argtypes = [C.c_int, C.c_void_p]
-        L.orc_two_body.argtypes = [C.c_double, _dp, C.c_double, _dp, _dp]
-        L.orc_battin.argtypes = [C.c_double, _dp, _dp, _dp]
-        _lib = L
+        _lib = bind(_LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def variant(path: str):
+    """Runs the wrapper's calls against another build of the oracle source
+    (a copy with a mistake planted in it, built by a test) for the duration
+    of the block; Field objects made inside it belong to that build."""
+    global _lib
+    saved = lib()
+    _lib = bind(path)
+    try:
+        yield _lib
+    finally:
+        _lib = saved
+
+
+def bind(path: str):
+    """Loads an oracle library and declares its C signatures."""
+    L = C.CDLL(path)
+    L.orc_field_create.restype = C.c_void_p
+    L.orc_field_create.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int]
+    L.orc_field_destroy.argtypes = [C.c_void_p]
+    L.orc_gravity_bf.argtypes = [C.c_void_p, _dp, _dp]
+    L.orc_gravity_bf_batch.argtypes = [C.c_void_p, _dp, C.c_int64, _dp, C.c_int]
+    L.orc_density.restype = C.c_double
+    L.orc_density.argtypes = [C.c_int, C.c_double]
+    L.orc_rhs.argtypes = [C.POINTER(Model), C.c_double, C.c_double, _dp, _dp]
+    L.orc_propagate.argtypes = [C.POINTER(Model), C.POINTER(Integ), C.c_double, _dp,
+                                C.c_double, C.c_double, _dp, C.POINTER(Stats)]
+    L.orc_propagate_batch.argtypes = [C.POINTER(Model), C.POINTER(Integ), _dp, _dp, _dp, _dp,
+                                      C.c_int64, _dp, _i32p, _i64p, _i64p, C.c_int]
+    L.orc_kep2eci.argtypes = [C.c_double, _dp, _dp]
+    L.orc_tableau.argtypes = [C.c_int, C.c_void_p]
+    L.orc_two_body.argtypes = [C.c_double, _dp, C.c_double, _dp, _dp]
+    L.orc_battin.argtypes = [C.c_double, _dp, _dp, _dp]
+    L.orc_propagate_sampled_batch.argtypes = [C.POINTER(Model), C.POINTER(Integ), _dp, C.c_double, _dp,
+                                              C.c_int64, _dp, C.c_int64, _dp, _i32p, _i64p, _i64p, C.c_int]
+    return L
 
 
 def _p(a):
@@ -199,6 +221,28 @@ def propagate_batch(model: Model, integ: Integ, y0, t0, tf, bc=None, threads=0):
                               _p(yf), status.ctypes.data_as(_i32p), steps.ctypes.data_as(_i64p),
                               evals.ctypes.data_as(_i64p), int(threads))
     return yf, status, steps, evals
+
+
+def propagate_sampled_batch(model: Model, integ: Integ, y0, t0, t_eval, bc=None, threads=0):
+    """Rows from a common t0 sampled at the increasing times t_eval (steps
+    shortened onto each sample, controller carried through them); returns
+    (states (n, m, 6), status, steps, evals)."""
+    y0 = np.ascontiguousarray(y0, dtype=np.float64).reshape(-1, 6)
+    n = y0.shape[0]
+    te = np.ascontiguousarray(t_eval, dtype=np.float64).reshape(-1)
+    m = te.shape[0]
+    bcp = None
+    if bc is not None:
+        bc = np.ascontiguousarray(np.broadcast_to(bc, (n,)), dtype=np.float64)
+        bcp = _p(bc)
+    out = np.empty((n, m, 6))
+    status = np.empty(n, dtype=np.int32)
+    steps = np.empty(n, dtype=np.int64)
+    evals = np.empty(n, dtype=np.int64)
+    lib().orc_propagate_sampled_batch(C.byref(model), C.byref(integ), _p(y0), float(t0), _p(te), m, bcp, n,
+                                      _p(out), status.ctypes.data_as(_i32p), steps.ctypes.data_as(_i64p),
+                                      evals.ctypes.data_as(_i64p), int(threads))
+    return out, status, steps, evals
 
 
 def kep2eci(mu, kep):

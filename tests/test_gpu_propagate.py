@@ -304,6 +304,42 @@ def test_graph_replay_matches_direct_launch():
             assert info["rows_ok"] == n
 
 
+def test_graph_after_sort_buffer_regrowth_matches_direct_launch():
+    """Regression: the captured sequence holds the hand-out tables' address.
+    A graph call at 1,000 rows, then a direct call at 100,000 rows (which
+    regrows and so frees that buffer), then the 1,000-row graph call again
+    must recapture (or replay into live memory) and give the direct launch's
+    bits, with the same per-row evaluation counts."""
+    announce("hipGraph after the hand-out buffer regrows: graph n=1000, direct n=100000, graph n=1000")
+    from bqp_amd.device import DeviceArray
+
+    small, big = 1000, 100000
+    w = inputs.workload("w1", n=big, seed=3)
+    field = bqp_amd.kaula(8)
+    tf_small = np.full(small, 1800.0)
+    with bqp_amd.Propagator(field, method="rk87", rtol=1e-11) as p:
+        ref, sref, iref = p.propagate_batch_final(w["y0"][:small], 0.0, tf_small)
+        y0 = DeviceArray.from_numpy(w["y0"][:small])
+        t0 = DeviceArray.from_numpy(np.zeros(small))
+        tfd = DeviceArray.from_numpy(tf_small)
+        ev = DeviceArray((small,), np.int32)
+        for rep in range(2):
+            p.set_launch(use_graph=1)
+            out, s, _ = p.propagate_batch_final_dev(y0, t0, tfd, None, None, None, None, ev)
+            assert np.array_equal(out.numpy(), ref), rep
+            assert np.array_equal(s.numpy(), sref)
+            assert np.array_equal(ev.numpy(), iref["evals_per_row"])
+            # a direct call on a larger batch regrows the hand-out buffer
+            p.set_launch(use_graph=0)
+            _, sb, _ = p.propagate_batch_final(w["y0"], 0.0, np.full(big, 300.0))
+            assert np.all(sb == 0)
+            p.set_launch(use_graph=1)
+            out, s, _ = p.propagate_batch_final_dev(y0, t0, tfd, None, None, None, None, ev)
+            assert np.array_equal(out.numpy(), ref), rep
+            assert np.array_equal(s.numpy(), sref)
+            assert np.array_equal(ev.numpy(), iref["evals_per_row"])
+
+
 def test_dealt_first_round_covers_every_row_once():
     """The first round of groups is dealt to the blocks by index (a
     back-and-forth deal over the hand-out order) and the queue starts after

@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 
 import bqp_amd
-from bqp_amd import OMEGA_EARTH, R_EARTH
+from bqp_amd import IN_SCOPE, OMEGA_EARTH, R_EARTH
 from gpu_util import announce
 
 pytestmark = pytest.mark.gpu
@@ -33,7 +33,7 @@ def _oracle(orc, s_km, span, gravity_order=5, tol=1e-8, am_ratio=0.01, cd=2.2, g
 def test_propagate_final_matches_oracle(orc):
     announce("propagate_final (km, GOCE d/o 5, exponential drag, Encke, DOP853) vs oracle")
     s = bqp_amd.kep_to_eci([6878.0, 0.001, 51.6, 10.0, 20.0, 30.0])
-    yk = bqp_amd.propagate_final(s, EPOCH, 6 * 3600.0)
+    yk = bqp_amd.propagate_final(s, EPOCH, 6 * 3600.0, **IN_SCOPE)
     yo, so = _oracle(orc, s, 6 * 3600.0)
     assert so[0] == 0
     assert np.linalg.norm(yk[:3] - yo[0, :3]) * 1e3 < 1e-2
@@ -43,7 +43,7 @@ def test_propagate_samples_match_final_and_oracle(orc):
     announce("propagate (sampled dict) vs propagate_final and the oracle")
     s = bqp_amd.kep_to_eci([7200.0, 0.01, 98.0, 40.0, 60.0, 0.0])
     tof = 4 * 3600.0
-    out = bqp_amd.propagate(s, EPOCH, tof, times_s=[600.0, 5400.0, 9000.0])
+    out = bqp_amd.propagate(s, EPOCH, tof, times_s=[600.0, 5400.0, 9000.0], **IN_SCOPE)
     assert set(out) == {"times_s", "states_eci", "completed", "terminal_event", "max_steps_exceeded", "steps",
                         "rhs_evals", "wall_us"}
     assert out["completed"] and not out["max_steps_exceeded"] and out["terminal_event"] == ""
@@ -53,7 +53,7 @@ def test_propagate_samples_match_final_and_oracle(orc):
     yo, so = _oracle(orc, s, tof, ground=False)
     assert so[0] == 0
     assert np.linalg.norm(out["states_eci"][-1, :3] - yo[0, :3]) * 1e3 < 1e-2
-    yf = bqp_amd.propagate_final(s, EPOCH, tof)
+    yf = bqp_amd.propagate_final(s, EPOCH, tof, **IN_SCOPE)
     assert np.linalg.norm(out["states_eci"][-1, :3] - yf[:3]) * 1e3 < 1e-2
 
 
@@ -63,7 +63,7 @@ def test_propagate_batch_final_matches_oracle_and_flags_impact(orc):
                     [26560.0, 0.01, 55.0, 200.0, 10.0, 300.0], [6500.0, 0.02, 10.0, 0.0, 0.0, 180.0]])
     s = np.array([bqp_amd.kep_to_eci(k) for k in kep])
     tof = np.array([3 * 3600.0, 5 * 3600.0, 12 * 3600.0, 3 * 3600.0])
-    yk, st = bqp_amd.propagate_batch_final(s, tof, epoch_jd=EPOCH)
+    yk, st = bqp_amd.propagate_batch_final(s, tof, epoch_jd=EPOCH, **IN_SCOPE)
     for k in range(3):
         yo_k, so_k = _oracle(orc, s[k:k + 1], tof[k])
         assert st[k] == 0 and so_k[0] == 0
@@ -72,7 +72,7 @@ def test_propagate_batch_final_matches_oracle_and_flags_impact(orc):
     # NaN-filled; the single-arc call reaches the ground event in flight
     assert st[3] == 6 and np.all(np.isnan(yk[3]))
     with pytest.raises(ValueError, match="ground"):
-        bqp_amd.propagate_final(s[3], EPOCH, tof[3])
+        bqp_amd.propagate_final(s[3], EPOCH, tof[3], **IN_SCOPE)
 
 
 def test_ground_event_over_the_ellipsoid_matches_oracle(orc):

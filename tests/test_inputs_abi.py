@@ -96,22 +96,58 @@ def test_icgem_gfc_parser(tmp_path):
     assert f.cbar[inputs.tri_index(2, 0)] == -4.84e-4 and f.sbar[inputs.tri_index(2, 2)] == -1.4e-6
 
 
+def test_reference_defaults_name_out_of_scope_models():
+    """The reference-shaped calls keep the reference's keyword defaults
+    (dust_transfer/src/lib.rs:238-241, 364-366: srp, sun, moon on, JB2008,
+    Vern7).  Those models are outside this propagator, so relying on any one
+    of them raises ValueError before any device work; the in-scope set
+    passes the model checks."""
+    import inspect
+
+    for fn in (bqp_amd.propagate, bqp_amd.propagate_final, bqp_amd.propagate_batch_final):
+        kw = {k: v.default for k, v in inspect.signature(fn).parameters.items()
+              if v.kind == inspect.Parameter.KEYWORD_ONLY}
+        assert kw["srp"] is True and kw["sun"] is True and kw["moon"] is True
+        assert kw["atm_model"] == 7 and kw["method"] == "vern7" and kw["gravity_order"] == 5
+        assert kw["drag"] is True and kw["tol"] == 1e-8 and kw["dt_max_s"] == 300.0
+    s = bqp_amd.kep_to_eci([7000.0, 0.001, 51.6, 10.0, 20.0, 30.0])
+    ok = bqp_amd.IN_SCOPE
+    with pytest.raises(ValueError, match="srp, sun and moon"):
+        bqp_amd.propagate_final(s, 2451545.0, 600.0)
+    with pytest.raises(ValueError, match="srp, sun and moon"):
+        bqp_amd.propagate(s, 2451545.0, 600.0)
+    with pytest.raises(ValueError, match="srp, sun and moon"):
+        bqp_amd.propagate_batch_final(s[None, :], [600.0])
+    for key in ("sun", "moon"):
+        with pytest.raises(ValueError, match="srp, sun and moon"):
+            bqp_amd.propagate_final(s, 2451545.0, 600.0, **{**ok, key: True})
+    with pytest.raises(ValueError, match="atm_model 7"):
+        bqp_amd.propagate_final(s, 2451545.0, 600.0, **{k: v for k, v in ok.items() if k != "atm_model"})
+    with pytest.raises(ValueError, match="vern7"):
+        bqp_amd.propagate_final(s, 2451545.0, 600.0, **{k: v for k, v in ok.items() if k != "method"})
+    # drag off: the atmosphere model is not consulted (as in the reference)
+    field, kw, _ = bqp_amd._model(5, False, False, False, False, 7, 0.01, 2.2, 1.3, 1e-8, 300.0, "dop853",
+                                  2451545.0, ground=True)
+    assert kw["drag"] == "none" and field.nmax == 5
+
+
 def test_python_boundary_validation_before_device():
     s = bqp_amd.kep_to_eci([7000.0, 0.001, 51.6, 10.0, 20.0, 30.0])
+    ok = bqp_amd.IN_SCOPE
     with pytest.raises(ValueError):
-        bqp_amd.propagate_final(s, 2451545.0, -1.0)
+        bqp_amd.propagate_final(s, 2451545.0, -1.0, **ok)
     with pytest.raises(ValueError):
-        bqp_amd.propagate_final(s, 2451545.0, 600.0, srp=True)
+        bqp_amd.propagate_final(s, 2451545.0, 600.0, **{**ok, "srp": True})
     with pytest.raises(ValueError):
-        bqp_amd.propagate_final(s, 2451545.0, 600.0, gravity_order=16)
+        bqp_amd.propagate_final(s, 2451545.0, 600.0, gravity_order=16, **ok)
     with pytest.raises(ValueError):
-        bqp_amd.propagate_final(s, 2451545.0, 600.0, method="vern7")
+        bqp_amd.propagate_final(s, 2451545.0, 600.0, **{**ok, "method": "vern7"})
     with pytest.raises(ValueError):
-        bqp_amd.propagate_final(s, 2451545.0, 600.0, atm_model=7)
+        bqp_amd.propagate_final(s, 2451545.0, 600.0, **{**ok, "atm_model": 7})
     with pytest.raises(ValueError):
-        bqp_amd.propagate_final([1, 2, np.nan, 4, 5, 6], 2451545.0, 600.0)
+        bqp_amd.propagate_final([1, 2, np.nan, 4, 5, 6], 2451545.0, 600.0, **ok)
     with pytest.raises(ValueError):
-        bqp_amd.propagate(s, 2451545.0, 600.0, times_s=[100.0, 50.0])
+        bqp_amd.propagate(s, 2451545.0, 600.0, times_s=[100.0, 50.0], **ok)
     # not a bound ellipse (above escape speed; radial, zero angular momentum)
     fast = s.copy()
     fast[3:] *= 1.5
@@ -187,3 +223,48 @@ def test_batch_min_radius_precheck():
     low_now = bqp_amd.kep_to_eci([6450.0, 0.0, 51.6, 0.0, 0.0, 0.0])
     ok = bqp_amd._clears_min_radius(np.array([good, low_perigee, low_now]), r_min)
     assert list(ok) == [True, False, False]
+
+
+class _StubBuf:
+    """A device-buffer stand-in: shape, dtype and a data_ptr() that must
+    never be reached (the contract check runs before any native call)."""
+
+    def __init__(self, shape, dtype):
+        self.shape, self.dtype = tuple(shape), np.dtype(dtype)
+
+    def data_ptr(self):
+        raise AssertionError("native call reached with a buffer that breaks the contract")
+
+
+def test_device_entry_point_rejects_bad_buffers_before_native_call():
+    """propagate_batch_final_dev takes raw device pointers; the kernel reads
+    y0/t0/tf/bc and writes yf/status/steps/evals as (n, 6) float64 and (n,)
+    float64 / int32 with no bounds of its own, so a wrong shape or dtype must
+    raise ValueError on the host.  The context is never touched: the check
+    runs before the ctypes call."""
+    from bqp_amd.propagator import Propagator
+
+    p = Propagator.__new__(Propagator)  # no context: any native call would fail
+    n = 8
+    good = dict(y0=_StubBuf((n, 6), np.float64), t0=_StubBuf((n,), np.float64), tf=_StubBuf((n,), np.float64),
+                bc=_StubBuf((n,), np.float64), yf=_StubBuf((n, 6), np.float64), status=_StubBuf((n,), np.int32),
+                steps=_StubBuf((n,), np.int32), evals=_StubBuf((n,), np.int32))
+    bad_cases = [
+        ("y0", _StubBuf((n * 6,), np.float64)),       # flat states
+        ("y0", _StubBuf((n, 6), np.float32)),
+        ("t0", _StubBuf((1,), np.float64)),           # a scalar time as a 1-element buffer
+        ("tf", _StubBuf((n,), np.int64)),
+        ("tf", _StubBuf((n + 1,), np.float64)),
+        ("bc", _StubBuf((n - 1,), np.float64)),
+        ("yf", _StubBuf((n, 3), np.float64)),
+        ("status", _StubBuf((n,), np.int64)),
+        ("steps", _StubBuf((n, 1), np.int32)),
+        ("evals", _StubBuf((n,), np.float32)),
+        ("t0", np.zeros(n)),                          # a host array, not a device buffer
+    ]
+    for name, buf in bad_cases:
+        kw = {**good, name: buf}
+        with pytest.raises(ValueError, match=name):
+            p.propagate_batch_final_dev(**kw)
+    with pytest.raises(ValueError, match="t0"):
+        p.propagate_batch_final_dev(good["y0"], None, good["tf"])
