@@ -82,6 +82,7 @@ def cpu_baseline(w, field, threads: int, target_s: float, days: float = 1.0) -> 
 
     pilot = min(n_all, 2 * threads)
     pw, _ = timed(pilot)
+    # whole rounds of `threads` rows (one row per thread at a time)
     rows = int(min(n_all, max(pilot, round(pilot * target_s / max(pw, 1e-3) / threads) * threads)))
     wall, ok = timed(rows)
     return {
@@ -90,7 +91,7 @@ def cpu_baseline(w, field, threads: int, target_s: float, days: float = 1.0) -> 
         "cores": threads,
         "kind": "port",
         "sample": f"first {rows} orbits of the same workload for {days:g} day(s) each ({ok}/{rows} ok), FP64 C "
-                  f"restatement of the same algorithm (oracle/), OpenMP dynamic schedule over rows, wall "
+                  f"restatement of the same algorithm (oracle/), OpenMP, one row per thread at a time, wall "
                   f"{wall:.1f} s (sized by a {pilot}-row pilot of {pw:.1f} s)",
     }
 

@@ -404,7 +404,44 @@ typedef struct rhs_ctx {
   int bad;   /* baseline failed */
 } rhs_ctx;
 
+/* Per-attempt trace of the calling thread's integrations (test hook): one
+ * record of ORC_TRACE_FIELDS doubles per step attempt. */
+static __thread double* tls_trace = NULL;
+static __thread int64_t tls_trace_cap = 0, tls_trace_n = 0;
+
+void orc_trace_begin(double* buf, int64_t cap_records) {
+  tls_trace = buf;
+  tls_trace_cap = cap_records;
+  tls_trace_n = 0;
+}
+
+int64_t orc_trace_end(void) {
+  const int64_t n = tls_trace_n;
+  tls_trace = NULL;
+  tls_trace_cap = tls_trace_n = 0;
+  return n;
+}
+
+static void trace_attempt(double t, double hs, double en, double thresh, int accepted, double h_next) {
+  if (!tls_trace) return;
+  if (tls_trace_n < tls_trace_cap) {
+    double* r = tls_trace + ORC_TRACE_FIELDS * tls_trace_n;
+    r[0] = t;
+    r[1] = hs;
+    r[2] = en;
+    r[3] = thresh;
+    r[4] = accepted ? 1.0 : 0.0;
+    r[5] = h_next;
+  }
+  ++tls_trace_n;
+}
+
 static void eval_rhs(rhs_ctx* c, double t, const double y[6], double dy[6]) {
+  if (c->md->formulation == ORC_TEST_EXP) { /* y' = y, solution y0 exp(t - t0) */
+    (void)t;
+    for (int i = 0; i < 6; ++i) dy[i] = y[i];
+    return;
+  }
   if (c->md->formulation != ORC_ENCKE) {
     orc_rhs(c->md, c->bc, t, y, dy);
     return;
@@ -606,6 +643,7 @@ static int integrate_segment(rhs_ctx* rc, const orc_integ* cfg, const bqp_tablea
       return loc->status;
     }
     const double tgt = (stops && stops->next < stops->m && stops->t[stops->next] < tb) ? stops->t[stops->next] : tb;
+    const double t_att = t;
     const double rem = tgt - t;
     double hs = (h < rem) ? h : rem;
     const int lands = (hs == rem);
@@ -718,6 +756,7 @@ static int integrate_segment(rhs_ctx* rc, const orc_integ* cfg, const bqp_tablea
       }
       fac = clampd(fac, 0.2, maxg);
       h = clampd(hs * fac, h_min, h_max);
+      trace_attempt(t_att, hs, en, thresh, 1, h);
       if (h_exit) *h_exit = h;
       err_prev = en;
       have_prev = 1;
@@ -753,6 +792,7 @@ static int integrate_segment(rhs_ctx* rc, const orc_integ* cfg, const bqp_tablea
         have_prev = 1;
       }
       just_rej = 1;
+      trace_attempt(t_att, hs, en, thresh, 0, h);
     }
   }
   (void)seg_rejects;
@@ -873,7 +913,7 @@ int orc_propagate_sampled_batch(const orc_model* md, const orc_integ* cfg, const
                                 int32_t* status, int64_t* steps, int64_t* evals, int threads) {
 #ifdef _OPENMP
   if (threads <= 0) threads = omp_get_max_threads();
-#pragma omp parallel for schedule(dynamic, 4) num_threads(threads)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
 #endif
   for (int64_t i = 0; i < n; ++i) {
     orc_stats st;
@@ -889,7 +929,9 @@ int orc_propagate_sampled_batch(const orc_model* md, const orc_integ* cfg, const
 }
 
 /* Ragged row-parallel batch (LY/src/session.rs:2325-2436): rows are
- * independent, results keyed by index, failed rows NaN-filled. */
+ * independent, results keyed by index, failed rows NaN-filled.  Rows are
+ * handed to threads one at a time (a row is milliseconds to seconds of
+ * work, so small samples still keep every thread busy). */
 int orc_propagate_batch(const orc_model* md, const orc_integ* cfg, const double* y0,
                         const double* t0, const double* tf, const double* bc, int64_t n,
                         double* yf, int32_t* status, int64_t* steps, int64_t* evals,
@@ -897,7 +939,7 @@ int orc_propagate_batch(const orc_model* md, const orc_integ* cfg, const double*
   int first_bad = 0;
 #ifdef _OPENMP
   if (threads <= 0) threads = omp_get_max_threads();
-#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
 #endif
   for (int64_t i = 0; i < n; ++i) {
     orc_stats st;

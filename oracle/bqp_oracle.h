@@ -90,6 +90,10 @@ typedef struct orc_model {
 /* Equations of motion: Cowell (inertial state) or Encke (deviation from a
  * two-body baseline, Battin's correction, rebased every ORC_ENCKE_SEGMENT_S). */
 enum { ORC_COWELL = 0, ORC_ENCKE = 1 };
+/* Test system for the step controller (not an orbit): y' = y in all six
+ * components, exact solution y0 exp(t - t0) (the reference's ExpSystem,
+ * LY/src/odesolve/basic_tests.rs). */
+#define ORC_TEST_EXP 2
 #define ORC_ENCKE_SEGMENT_S 5400.0
 
 typedef struct orc_integ {
@@ -136,6 +140,15 @@ int orc_propagate_batch(const orc_model* m, const orc_integ* cfg, const double* 
                         const double* t0, const double* tf, const double* bc, int64_t n,
                         double* yf, int32_t* status, int64_t* steps, int64_t* evals,
                         int threads);
+
+/* Trace of the calling thread's step attempts (test hook): after
+ * orc_trace_begin(buf, cap), every attempt of orc_propagate on this thread
+ * appends ORC_TRACE_FIELDS doubles {t, h, err_norm, threshold, accepted
+ * (1/0), next h} until cap records; orc_trace_end returns the number of
+ * attempts made (which may exceed cap) and stops tracing. */
+#define ORC_TRACE_FIELDS 6
+void orc_trace_begin(double* buf, int64_t cap_records);
+int64_t orc_trace_end(void);
 
 void orc_kep2eci(double mu, const double kep[6], double eci[6]);
 

@@ -122,6 +122,8 @@ def bind(path: str):
     L.orc_tableau.argtypes = [C.c_int, C.c_void_p]
     L.orc_two_body.argtypes = [C.c_double, _dp, C.c_double, _dp, _dp]
     L.orc_battin.argtypes = [C.c_double, _dp, _dp, _dp]
+    L.orc_trace_begin.argtypes = [_dp, C.c_int64]
+    L.orc_trace_end.restype = C.c_int64
     L.orc_propagate_sampled_batch.argtypes = [C.POINTER(Model), C.POINTER(Integ), _dp, C.c_double, _dp,
                                               C.c_int64, _dp, C.c_int64, _dp, _i32p, _i64p, _i64p, C.c_int]
     return L
@@ -164,7 +166,7 @@ def density(model: str, h_m: float) -> float:
     return lib().orc_density(DRAG[model], float(h_m))
 
 
-FORMULATION = {"cowell": 0, "encke": 1}
+FORMULATION = {"cowell": 0, "encke": 1, "test_exp": 2}
 
 
 def make_model(field: Field, omega=7.292115e-5, theta0=0.0, drag="none", r_atm=6378137.0,
@@ -202,6 +204,21 @@ def propagate(model: Model, integ: Integ, y0, t0, tf, bc=0.0):
     lib().orc_propagate(C.byref(model), C.byref(integ), float(bc), _p(y0), float(t0), float(tf),
                         _p(yf), C.byref(st))
     return yf, st
+
+
+def propagate_traced(model: Model, integ: Integ, y0, t0, tf, bc=0.0, cap=200000):
+    """orc_propagate with the per-attempt trace on: returns (yf, stats,
+    trace) with trace an (attempts, 6) array of t, h, err_norm, threshold,
+    accepted, next h."""
+    buf = np.zeros((cap, 6))
+    lib().orc_trace_begin(_p(buf), cap)
+    try:
+        yf, st = propagate(model, integ, y0, t0, tf, bc)
+    finally:
+        n = lib().orc_trace_end()
+    if n > cap:
+        raise ValueError(f"trace overflow: {n} attempts > cap {cap}")
+    return yf, st, buf[:n].copy()
 
 
 def propagate_batch(model: Model, integ: Integ, y0, t0, tf, bc=None, threads=0):

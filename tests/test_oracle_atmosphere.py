@@ -26,10 +26,10 @@ place of v_rel each fail at least one of them.
 import json
 import os
 import shutil
-import subprocess
 
 import numpy as np
 import pytest
+from oracle_variants import build_variant
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -168,7 +168,6 @@ def test_oracle_drag_energy_decay_matches_quadrature(orc):
 
 # ------------------------------------------------------------ planted mistakes
 
-_SRC = os.path.join(ROOT, "oracle", "bqp_oracle.c")
 PLANTS = {
     # one digit of the 400-450 km band's nominal density (3.725e-12 -> 3.752e-12)
     "table_digit": ("3.725e-12", "3.752e-12"),
@@ -180,24 +179,10 @@ PLANTS = {
 }
 
 
-def _build_variant(name, tmp):
-    old, new = PLANTS[name]
-    src = open(_SRC).read()
-    assert src.count(old) == 1, f"planting {name}: pattern not found once in the oracle source"
-    path = os.path.join(tmp, f"oracle_{name}.c")
-    with open(path, "w") as fh:
-        fh.write(src.replace(old, new))
-    lib = os.path.join(tmp, f"liboracle_{name}.so")
-    subprocess.run(["gcc", "-O2", "-march=x86-64-v3", "-ffp-contract=off", "-fno-math-errno", "-fopenmp", "-fPIC",
-                    "-std=gnu11", "-I", os.path.join(ROOT, "oracle"), "-shared", "-o", lib, path, "-lm"],
-                   check=True)
-    return lib
-
-
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc to build the planted variants")
 @pytest.mark.parametrize("name", sorted(PLANTS))
 def test_planted_mistake_fails_a_pin(orc, tmp_path, name):
-    lib = _build_variant(name, str(tmp_path))
+    lib = build_variant(name, *PLANTS[name], str(tmp_path))
     with orc.variant(lib):
         dens = _density_checks(orc)
         term = _drag_term_checks(orc)
