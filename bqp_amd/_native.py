@@ -110,6 +110,7 @@ EXPORTS = [
     "bqp_gravity_accel_batch_dev",
     "bqp_propagate_batch_final",
     "bqp_propagate_batch_final_dev",
+    "bqp_propagate_batch_final_dev_noalloc",
     "bqp_propagate_batch_sampled",
     "bqp_get_tableau",
     "bqp_flops_per_rhs",
@@ -160,6 +161,8 @@ def load(path: str | None = None):
     L.bqp_propagate_batch_final.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int64, _dp, _i32p, _i32p,
                                             _i32p, C.POINTER(Stats)]
     L.bqp_propagate_batch_final_dev.argtypes = [C.c_void_p] + [C.c_void_p] * 4 + [C.c_int64] + \
+        [C.c_void_p] * 4 + [C.POINTER(Stats)]
+    L.bqp_propagate_batch_final_dev_noalloc.argtypes = [C.c_void_p] + [C.c_void_p] * 4 + [C.c_int64] + \
         [C.c_void_p] * 4 + [C.POINTER(Stats)]
     L.bqp_propagate_batch_sampled.argtypes = [C.c_void_p, _dp, C.c_double, _dp, C.c_int64, _dp, C.c_int64,
                                               _dp, _i32p, C.POINTER(Stats)]

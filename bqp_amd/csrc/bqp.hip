@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -21,6 +22,51 @@
 #include "prop_forms.hpp"
 
 using namespace bqp;
+
+// ---------------------------------------------------------------- static scratch
+// Device memory of the no-allocation entry point
+// (bqp_propagate_batch_final_dev_noalloc): one array per GPU, held until the
+// process ends.  Layout for a call of n <= BQP_NOALLOC_MAX_ROWS rows:
+//   [0, 256)        work-queue head (u32)
+//   [256, 512)      row totals (5 x u64)
+//   [512, 1024)     first failed row (i32)
+//   [1024, ...)     hand-out tables: histogram (SORT_BUCKETS x u32) | order (n x i32) | bucket (n x u16)
+//   NOALLOC_ROWS_OFF  per-row status | steps | evals | rejects (4 x n x i32)
+// Host code never names `scratch` as an address: hipGetSymbolAddress gives
+// each device's copy (noalloc_base), and every launch takes that pointer.
+constexpr size_t NOALLOC_SORT_OFF = 1024;
+constexpr size_t NOALLOC_ROWS_OFF =
+    (NOALLOC_SORT_OFF + (size_t)SORT_BUCKETS * 4 + (size_t)BQP_NOALLOC_MAX_ROWS * 6 + 255) / 256 * 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_ROWS_OFF + (size_t)BQP_NOALLOC_MAX_ROWS * 16;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+static_assert(NOALLOC_BYTES <= BQP_NOALLOC_SCRATCH_BYTES, "include/bqp.h states the scratch size per GPU");
+static_assert(SORT_BUCKETS == 4096, "BQP_NOALLOC_SCRATCH_BYTES counts 4096 histogram buckets");
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_base[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of this device's scratch array, looked up once per device
+// with that device current (called from bqp_create).  nullptr on failure.
+void* noalloc_base(int dev) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_base[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return nullptr;
+    g_noalloc_base[dev] = ptr;
+  }
+  return g_noalloc_base[dev];
+}
+
+// The address looked up at set-up, without a new lookup.
+void* noalloc_base_known(int dev) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  return g_noalloc_base[dev];
+}
+}  // namespace
 
 // Identity of a captured launch sequence: a replay is valid only for the same
 // buffers, sizes, settings and field.
@@ -290,14 +336,27 @@ int reset_cost_model(bqp_ctx* c) {
   return hipGetLastError() == hipSuccess ? BQP_OK : BQP_E_DEVICE;
 }
 
+// Device memory a propagation call uses beyond its inputs and outputs: the
+// work-queue head, the row totals, the first failed row and the hand-out
+// tables (histogram | order | bucket).  The context's own buffers, or the
+// library's static scratch array (the no-allocation entry point).
+struct CallScratch {
+  unsigned int* queue;
+  unsigned long long* totals;
+  int* first;
+  void* sort;
+};
+
+CallScratch context_scratch(bqp_ctx* c) { return CallScratch{c->d_queue, c->d_totals, c->d_first, c->d_sort}; }
+
 // Enqueues the whole call (order, queue reset, persistent kernel, row
 // statistics) on the context stream.  Pure kernel launches: no allocation,
 // no memset, no synchronisation, so the sequence can be captured in a graph.
 void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, prop_fn kern, int G, int blocks,
-                         bool sort, bool events) {
+                         bool sort, bool events, const CallScratch& sc) {
   PropParams q = p;
   if (sort) {
-    char* base = (char*)c->d_sort;
+    char* base = (char*)sc.sort;
     unsigned int* hist = (unsigned int*)base;
     int32_t* order = (int32_t*)(base + SORT_BUCKETS * sizeof(unsigned int));
     unsigned short* bucket = (unsigned short*)(base + SORT_BUCKETS * sizeof(unsigned int) + n * sizeof(int32_t));
@@ -312,17 +371,17 @@ void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, prop_fn ker
     q.order = order;
   }
   q.deal_cus = c->cus;
-  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue, queue_start(n, blocks));
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, sc.queue, queue_start(n, blocks));
   if (p.sur_totals) hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, p.sur_totals, 5);
   if (events) (void)hipEventRecord(c->ev0, c->stream);
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * G), 0, c->stream, q);
   if (events) (void)hipEventRecord(c->ev1, c->stream);
-  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_totals, 5);
-  hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, c->stream, c->d_first, 0x7fffffff);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, sc.totals, 5);
+  hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, c->stream, sc.first, 0x7fffffff);
   hipLaunchKernelGGL(row_stats_kernel, dim3((unsigned)std::min<int64_t>(1024, (n + 255) / 256)), dim3(256), 0,
-                     c->stream, q.status, q.steps, q.evals, q.rejects, (int)n, c->d_totals);
+                     c->stream, q.status, q.steps, q.evals, q.rejects, (int)n, sc.totals);
   hipLaunchKernelGGL(first_failed_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, q.status,
-                     (int)n, c->d_first);
+                     (int)n, sc.first);
   enqueue_cost_learn(c, p, n);
 }
 
@@ -467,8 +526,11 @@ int run_surrogate_partitioned(bqp_ctx* c, PropParams p, int64_t n, int kprec, in
   return BQP_OK;
 }
 
-// Runs one propagation over device buffers; fills per-row outputs.
-int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
+// Runs one propagation over device buffers; fills per-row outputs.  With
+// `fixed` the call uses that scratch and allocates nothing: direct launches
+// only (no graph), and a checked surrogate keeps its checks in the kernel
+// instead of the up-front partition.
+int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats, const CallScratch* fixed) {
   if (c->nmax < 0 || !c->d_tab) return fail(c, BQP_E_NO_FIELD, "no gravity field loaded (call bqp_load_field)");
   if (n <= 0) return BQP_OK;
   if (n > 0x7fffffffLL) return fail(c, BQP_E_INVALID, "n exceeds 2^31-1 rows");
@@ -491,14 +553,15 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
 
   // all allocation happens here, before any launch is enqueued or captured
   const bool sort = c->launch.sort_by_cost != 0 && p.t_eval == nullptr && n > 64;
-  if (sort) {
+  if (sort && !fixed) {
     const size_t need = (size_t)SORT_BUCKETS * sizeof(unsigned int) + (size_t)n * sizeof(int32_t) +
                         (size_t)n * sizeof(unsigned short) + 512;
     int rc = ensure(c, &c->d_sort, &c->sort_bytes, need);
     if (rc) return rc;
   }
+  const CallScratch sc = fixed ? *fixed : context_scratch(c);
   p.order = nullptr;
-  p.queue = c->d_queue;
+  p.queue = sc.queue;
   p.n = (int)n;
   p.field = field_view(c);
   p.mu = c->mu;
@@ -541,7 +604,7 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
     p.sur_totals = nullptr;
   }
 
-  if (sur && c->mlp->check_every > 0 && !c->launch.use_graph)
+  if (sur && c->mlp->check_every > 0 && !c->launch.use_graph && !fixed)
     return run_surrogate_partitioned(c, p, n, kprec, G, kern, stats);
 
   const int64_t groups = (n + 63) / 64;
@@ -554,7 +617,7 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
   }
   blocks = (int)std::max<int64_t>(1, std::min<int64_t>(blocks, groups));
 
-  if (c->launch.use_graph) {
+  if (c->launch.use_graph && !fixed) {
     GraphKey key;
     std::memset(&key, 0, sizeof(key));
     key.p = p;
@@ -572,7 +635,7 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
     if (!c->graph_exec || !(key == *c->graph_key)) {
       bqp_drop_graph(c);
       HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_propagation(c, p, n, kern, G, blocks, sort, false);
+      enqueue_propagation(c, p, n, kern, G, blocks, sort, false, sc);
       hipGraph_t graph = nullptr;
       const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
       if (ce != hipSuccess || !graph) return fail(c, BQP_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
@@ -589,14 +652,14 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats) {
     HIP_TRY(c, hipGraphLaunch(c->graph_exec, c->stream));
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
   } else {
-    enqueue_propagation(c, p, n, kern, G, blocks, sort, true);
+    enqueue_propagation(c, p, n, kern, G, blocks, sort, true, sc);
   }
   HIP_TRY(c, hipGetLastError());
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   unsigned long long tot[5] = {0, 0, 0, 0, 0};
   int first = 0x7fffffff;
-  HIP_TRY(c, hipMemcpy(tot, c->d_totals, sizeof(tot), hipMemcpyDeviceToHost));
-  HIP_TRY(c, hipMemcpy(&first, c->d_first, sizeof(int), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(tot, sc.totals, sizeof(tot), hipMemcpyDeviceToHost));
+  HIP_TRY(c, hipMemcpy(&first, sc.first, sizeof(int), hipMemcpyDeviceToHost));
   if (stats) {
     float ms = 0.0f;
     HIP_TRY(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
@@ -691,6 +754,9 @@ int32_t bqp_create(const bqp_config* cfg, bqp_ctx** out) {
   }
   c->cus = prop.multiProcessorCount;
   c->launch.sort_by_cost = 1;
+  // the no-allocation entry point's scratch address on this device (the
+  // lookup allocates nothing; a failure only disables that entry point)
+  (void)noalloc_base(dev);
   {
     static const bqp_tableau tabs[3] = {BQP_DOPRI5_TABLEAU, BQP_DOP853_TABLEAU, BQP_RK87_TABLEAU};
     SecondOrderTableau so[3];
@@ -909,7 +975,40 @@ int32_t bqp_propagate_batch_final_dev(bqp_ctx* c, const double* d_y0, const doub
   p.steps = d_steps ? d_steps : s0 + n;
   p.evals = d_evals ? d_evals : s0 + 2 * n;
   p.rejects = s0 + 3 * n;
-  return run_propagation(c, p, n, stats);
+  return run_propagation(c, p, n, stats, nullptr);
+}
+
+int32_t bqp_propagate_batch_final_dev_noalloc(bqp_ctx* c, const double* d_y0, const double* d_t0,
+                                              const double* d_tf, const double* d_bc, int64_t n, double* d_yf,
+                                              int32_t* d_status, int32_t* d_steps, int32_t* d_evals,
+                                              bqp_stats* stats) {
+  if (!c) return BQP_E_INVALID;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (n < 0 || (n > 0 && (!d_y0 || !d_t0 || !d_tf || !d_yf)))
+    return fail(c, BQP_E_INVALID, "y0, t0, tf and yf must not be NULL");
+  if (n > BQP_NOALLOC_MAX_ROWS)
+    return fail(c, BQP_E_INVALID, "n = " + std::to_string(n) + " exceeds the no-allocation entry point's maximum of " +
+                                      std::to_string(BQP_NOALLOC_MAX_ROWS) + " rows (use bqp_propagate_batch_final_dev)");
+  if (n == 0) return BQP_OK;
+  char* base = (char*)noalloc_base_known(c->device);
+  if (!base) return fail(c, BQP_E_DEVICE, "the static scratch array's device address is unavailable");
+  CallScratch sc;
+  sc.queue = (unsigned int*)base;
+  sc.totals = (unsigned long long*)(base + 256);
+  sc.first = (int*)(base + 512);
+  sc.sort = base + NOALLOC_SORT_OFF;
+  int32_t* rows = (int32_t*)(base + NOALLOC_ROWS_OFF);
+  PropParams p{};
+  p.y0 = d_y0;
+  p.t0 = d_t0;
+  p.tf = d_tf;
+  p.bc = d_bc;
+  p.yf = d_yf;
+  p.status = d_status ? d_status : rows;
+  p.steps = d_steps ? d_steps : rows + n;
+  p.evals = d_evals ? d_evals : rows + 2 * n;
+  p.rejects = rows + 3 * n;
+  return run_propagation(c, p, n, stats, &sc);
 }
 
 int32_t bqp_propagate_batch_final(bqp_ctx* c, const double* y0, const double* t0, const double* tf,
@@ -951,7 +1050,7 @@ int32_t bqp_propagate_batch_final(bqp_ctx* c, const double* y0, const double* t0
   p.steps = dsp;
   p.evals = dev;
   p.rejects = drj;
-  const int rc2 = run_propagation(c, p, n, stats);
+  const int rc2 = run_propagation(c, p, n, stats, nullptr);
   if (rc2 != BQP_OK && rc2 != BQP_E_ROWS_FAILED) return rc2;
   HIP_TRY(c, hipMemcpy(yf, dyf, d6, hipMemcpyDeviceToHost));
   if (row_status) HIP_TRY(c, hipMemcpy(row_status, dst, i1, hipMemcpyDeviceToHost));
@@ -1003,7 +1102,7 @@ int32_t bqp_propagate_batch_sampled(bqp_ctx* c, const double* y0, double t0, con
   p.steps = dsp;
   p.evals = dev;
   p.rejects = drj;
-  const int rc2 = run_propagation(c, p, n, stats);
+  const int rc2 = run_propagation(c, p, n, stats, nullptr);
   if (rc2 != BQP_OK && rc2 != BQP_E_ROWS_FAILED) return rc2;
   HIP_TRY(c, hipMemcpy(out, dout, ds, hipMemcpyDeviceToHost));
   if (row_status) HIP_TRY(c, hipMemcpy(row_status, dst, i1, hipMemcpyDeviceToHost));

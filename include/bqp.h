@@ -171,6 +171,28 @@ int32_t bqp_propagate_batch_final_dev(bqp_ctx* ctx, const double* d_y0, const do
                                       double* d_yf, int32_t* d_row_status, int32_t* d_row_steps,
                                       int32_t* d_row_evals, bqp_stats* stats);
 
+/* Same as bqp_propagate_batch_final_dev, allocating nothing: no hipMalloc,
+ * hipFree or allocating library call, for callers that cannot allocate
+ * (code under HIP-graph capture, tight serving loops, harnesses that forbid
+ * it).  No counterpart in the reference.  Its work queue, totals, hand-out
+ * tables and per-row counters live in a static device array of the library
+ * (BQP_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends), so:
+ *   - n must be at most BQP_NOALLOC_MAX_ROWS (BQP_E_INVALID otherwise);
+ *   - calls that overlap on one GPU (two contexts, two threads, two streams)
+ *     race on that array: serialise them;
+ *   - every launch is direct (bqp_launch.use_graph is ignored), and a checked
+ *     surrogate keeps all its checks in the kernel (no up-front partition);
+ *     results equal bqp_propagate_batch_final_dev's.
+ * The context must come from bqp_create (which looks the array's address
+ * up once per device, with the device current); the call runs on the
+ * context's device and stream. */
+#define BQP_NOALLOC_MAX_ROWS 262144
+#define BQP_NOALLOC_SCRATCH_BYTES (1024 + 4096 * 4 + BQP_NOALLOC_MAX_ROWS * 22 + 256)
+int32_t bqp_propagate_batch_final_dev_noalloc(bqp_ctx* ctx, const double* d_y0, const double* d_t0,
+                                              const double* d_tf, const double* d_bc, int64_t n, double* d_yf,
+                                              int32_t* d_row_status, int32_t* d_row_steps,
+                                              int32_t* d_row_evals, bqp_stats* stats);
+
 /* Fixed-grid sampled batch: every row starts at t0 and is sampled at the
  * m increasing times t_eval[0..m-1] (>= t0), output (n, m, 6) inertial states.
  * Samples are hit exactly (the step is shortened onto each sample time). */
