@@ -42,7 +42,7 @@ from .surrogate import Surrogate
 __all__ = [
     "Propagator", "Surrogate", "Field", "BqpError", "kaula", "goce", "load_icgem", "tableau", "inputs", "elements",
     "kep_to_eci", "eci_to_kep", "propagate", "propagate_final", "propagate_batch_final", "gmst",
-    "ROW_STATUS", "IN_SCOPE",
+    "ROW_STATUS", "IN_SCOPE", "epoch_offsets",
 ]
 
 MU_KM = 398600.4415  # km^3/s^2, the reference's constant (satpy_core/src/lib.rs:74)
@@ -234,19 +234,52 @@ def propagate(state_eci, epoch_jd, tof_s, times_s=None, *, gravity_order=5, drag
     }
 
 
+def epoch_offsets(epoch_jd, omega: float = OMEGA_EARTH):
+    """Per-row clock offsets (s) that give each row its own epoch in one
+    batch whose body frame turns uniformly, theta(t) = theta0 + omega t,
+    with theta0 = gmst(epoch_jd[0]): row i starts at t = offset[i], where
+    theta0 + omega offset[i] = gmst(epoch_jd[i]) (mod 2 pi), offset in
+    [0, 2 pi / omega).  Returns (theta0, offsets)."""
+    ep = np.asarray(epoch_jd, dtype=np.float64).reshape(-1)
+    theta0 = gmst(float(ep[0]))
+    d = np.mod(np.array([gmst(float(e)) for e in ep]) - theta0, 2.0 * math.pi)
+    return theta0, d / omega
+
+
 def propagate_batch_final(states_eci, tof_s, *, epoch_jd=2451545.0, gravity_order=5, drag=True, srp=True,
                           sun=True, moon=True, atm_model=7, am_ratio=0.01, cd=2.2, cr=1.3, tol=1e-8,
                           dt_max_s=300.0, method="vern7"):
     """Batch of ECI states (n, 6) km with per-row tof (n,) s -> (states (n, 6) km, status (n,)).
-    Failed rows are NaN-filled; ``status`` codes are in ``ROW_STATUS``."""
+
+    The reference's ragged batch (``VariableFinalBatchRequest``,
+    LY/src/session.rs:2028-2035, row worker :2803-2873): ``epoch_jd``,
+    ``am_ratio`` and ``cd`` may be scalars or per-row arrays (n,).  Row i runs
+    from its epoch for tof_s[i] seconds; its body frame starts at
+    gmst(epoch_jd[i]) (per-row clock offsets, ``epoch_offsets``).  Failed
+    rows are NaN-filled; ``status`` codes are in ``ROW_STATUS``."""
     y = np.asarray(states_eci, dtype=np.float64).reshape(-1, 6)
-    tof = np.broadcast_to(np.asarray(tof_s, dtype=np.float64), (y.shape[0],))
+    n = y.shape[0]
+    tof = np.broadcast_to(np.asarray(tof_s, dtype=np.float64), (n,))
     if not np.all(np.isfinite(tof)) or np.any(tof < 0):
         raise ValueError("tof_s must be finite and >= 0")
-    field, kw, bc = _model(gravity_order, drag, srp, sun, moon, atm_model, am_ratio, cd, cr, tol, dt_max_s,
-                           method, epoch_jd, ground=True)
+    try:
+        ep = np.ascontiguousarray(np.broadcast_to(np.asarray(epoch_jd, dtype=np.float64), (n,)))
+        am = np.ascontiguousarray(np.broadcast_to(np.asarray(am_ratio, dtype=np.float64), (n,)))
+        cdv = np.ascontiguousarray(np.broadcast_to(np.asarray(cd, dtype=np.float64), (n,)))
+    except ValueError:
+        raise ValueError("epoch_jd, am_ratio and cd must be scalars or have one value per row") from None
+    if not np.all(np.isfinite(ep)):
+        raise ValueError("epoch_jd must be finite")
+    if not (np.all(np.isfinite(am)) and np.all(am > 0) and np.all(np.isfinite(cdv)) and np.all(cdv > 0)):
+        raise ValueError("am_ratio and cd must be finite and > 0 in every row")
+    if n == 0:
+        return np.empty((0, 6)), np.empty(0, dtype=np.int32)
+    field, kw, _ = _model(gravity_order, drag, srp, sun, moon, atm_model, float(am[0]), float(cdv[0]), cr, tol,
+                          dt_max_s, method, float(ep[0]), ground=True)
+    theta0, off = epoch_offsets(ep, kw["omega"])
+    kw["theta0"] = theta0
     with Propagator(field, **kw) as p:
-        yf, status, _ = p.propagate_batch_final(y * 1e3, 0.0, tof, np.full(y.shape[0], bc) if drag else None)
+        yf, status, _ = p.propagate_batch_final(y * 1e3, off, off + tof, am * cdv if drag else None)
     # rows whose initial radius or osculating perigee is below R + 100 km are
     # refused up front, as the reference's batch does (session.rs:1983-1995,
     # 2140-2144): status below_min_radius, NaN-filled

@@ -90,3 +90,30 @@ def test_ground_event_over_the_ellipsoid_matches_oracle(orc):
         _, so, _, _ = orc.propagate_batch(m, i, polar, 0.0, 60.0)
         assert int(st[0]) == want and int(so[0]) == want
 
+
+
+def test_batch_rows_with_own_epochs_and_ballistics_equal_separate_calls():
+    """The reference's ragged batch carries an epoch and ballistic
+    coefficients per row (VariableFinalBatchRequest, LY/src/session.rs:
+    2028-2035).  Rows with different epochs (spread over three days) and
+    different A/m and Cd in one call agree with one call per row at that
+    row's epoch and ballistics; the per-row clock offsets only move each
+    row's time origin."""
+    announce("propagate_batch_final: per-row epochs and ballistics vs one call per row")
+    kep = np.array([[6778.0, 0.001, 51.6, 0.0, 0.0, 0.0], [7000.0, 0.002, 28.5, 100.0, 30.0, 45.0],
+                    [6900.0, 0.01, 97.8, 200.0, 10.0, 300.0], [7100.0, 0.005, 63.4, 10.0, 80.0, 120.0]])
+    s = np.array([bqp_amd.kep_to_eci(k) for k in kep])
+    tof = np.array([3 * 3600.0, 5 * 3600.0, 4 * 3600.0, 2 * 3600.0])
+    epochs = np.array([2460000.5, 2460001.37, 2459998.81, 2460002.05])
+    am = np.array([0.01, 0.02, 0.005, 0.015])
+    cd = np.array([2.2, 2.0, 2.4, 2.3])
+    yk, st = bqp_amd.propagate_batch_final(s, tof, epoch_jd=epochs, am_ratio=am, cd=cd, **IN_SCOPE)
+    assert np.all(st == 0)
+    for i in range(4):
+        yi, si = bqp_amd.propagate_batch_final(s[i:i + 1], tof[i:i + 1], epoch_jd=epochs[i], am_ratio=am[i],
+                                               cd=cd[i], **IN_SCOPE)
+        assert si[0] == 0
+        assert np.linalg.norm(yk[i, :3] - yi[0, :3]) * 1e3 < 1e-3
+    # the epoch matters: the same rows at one common epoch end elsewhere
+    y0, _ = bqp_amd.propagate_batch_final(s, tof, epoch_jd=epochs[0], am_ratio=am, cd=cd, **IN_SCOPE)
+    assert np.all(np.linalg.norm(yk[1:, :3] - y0[1:, :3], axis=1) * 1e3 > 1e-3)

@@ -268,3 +268,28 @@ def test_device_entry_point_rejects_bad_buffers_before_native_call():
             p.propagate_batch_final_dev(**kw)
     with pytest.raises(ValueError, match="t0"):
         p.propagate_batch_final_dev(good["y0"], None, good["tf"])
+
+
+def test_epoch_offsets_reproduce_each_rows_frame_angle():
+    """Row i of a batch starts at t = offset[i] of a frame turning as
+    theta0 + omega t; there its angle equals gmst(epoch_jd[i])."""
+    ep = np.array([2451545.0, 2451545.25, 2451547.7, 2451544.9, 2460000.5])
+    theta0, off = bqp_amd.epoch_offsets(ep)
+    assert theta0 == bqp_amd.gmst(ep[0]) and off[0] == 0.0
+    assert np.all(off >= 0.0) and np.all(off < 2 * np.pi / bqp_amd.OMEGA_EARTH)
+    for e, o in zip(ep, off):
+        d = (theta0 + bqp_amd.OMEGA_EARTH * o - bqp_amd.gmst(e) + np.pi) % (2 * np.pi) - np.pi
+        assert abs(d) < 1e-12
+
+
+def test_per_row_batch_arguments_are_validated():
+    s = np.array([bqp_amd.kep_to_eci([7000.0, 0.001, 51.6, 10.0, 20.0, 30.0])] * 3)
+    ok = bqp_amd.IN_SCOPE
+    with pytest.raises(ValueError, match="one value per row"):
+        bqp_amd.propagate_batch_final(s, 600.0, epoch_jd=[2451545.0, 2451546.0], **ok)
+    with pytest.raises(ValueError, match="every row"):
+        bqp_amd.propagate_batch_final(s, 600.0, am_ratio=[0.01, -0.01, 0.01], **ok)
+    with pytest.raises(ValueError, match="every row"):
+        bqp_amd.propagate_batch_final(s, 600.0, cd=[2.2, np.nan, 2.2], **ok)
+    with pytest.raises(ValueError, match="finite"):
+        bqp_amd.propagate_batch_final(s, 600.0, epoch_jd=[2451545.0, np.inf, 2451545.0], **ok)
