@@ -94,6 +94,7 @@ class Launch(C.Structure):
         ("use_graph", C.c_int32),
         ("blocks", C.c_int32),
         ("sort_by_cost", C.c_int32),
+        ("teams_per_block", C.c_int32),
     ]
 
 

@@ -74,7 +74,7 @@ struct GraphKey {
   PropParams p;
   int64_t n;
   bqp_config cfg;
-  int G, blocks, sort;
+  int G, teams, blocks, sort;
   const void* tab;
   const void* sort_buf;  // the hand-out tables' buffer (hist, order, bucket), not in p
   bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
@@ -336,6 +336,16 @@ int reset_cost_model(bqp_ctx* c) {
   return hipGetLastError() == hipSuccess ? BQP_OK : BQP_E_DEVICE;
 }
 
+// Kernel and launch shape of one propagation over n rows.
+struct LaunchShape {
+  prop_fn kern = nullptr;
+  int threads = 0;       // workgroup size
+  int blocks = 0;        // persistent grid
+  int G = 0;             // fixed-G kernel: waves per group (0 for the team kernel)
+  int teams = 0;         // team kernel: groups per workgroup (0 for the fixed-G kernel)
+  unsigned queue0 = 0;   // first queue position: the groups of the dealt first round come before it
+};
+
 // Device memory a propagation call uses beyond its inputs and outputs: the
 // work-queue head, the row totals, the first failed row and the hand-out
 // tables (histogram | order | bucket).  The context's own buffers, or the
@@ -352,8 +362,8 @@ CallScratch context_scratch(bqp_ctx* c) { return CallScratch{c->d_queue, c->d_to
 // Enqueues the whole call (order, queue reset, persistent kernel, row
 // statistics) on the context stream.  Pure kernel launches: no allocation,
 // no memset, no synchronisation, so the sequence can be captured in a graph.
-void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, prop_fn kern, int G, int blocks,
-                         bool sort, bool events, const CallScratch& sc) {
+void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, const LaunchShape& ls, bool sort,
+                         bool events, const CallScratch& sc) {
   PropParams q = p;
   if (sort) {
     char* base = (char*)sc.sort;
@@ -371,10 +381,11 @@ void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, prop_fn ker
     q.order = order;
   }
   q.deal_cus = c->cus;
-  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, sc.queue, queue_start(n, blocks));
+  q.teams = ls.teams;
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, sc.queue, ls.queue0);
   if (p.sur_totals) hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, p.sur_totals, 5);
   if (events) (void)hipEventRecord(c->ev0, c->stream);
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * G), 0, c->stream, q);
+  hipLaunchKernelGGL(ls.kern, dim3(ls.blocks), dim3(ls.threads), 0, c->stream, q);
   if (events) (void)hipEventRecord(c->ev1, c->stream);
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, sc.totals, 5);
   hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(1), 0, c->stream, sc.first, 0x7fffffff);
@@ -410,15 +421,44 @@ int grid_for(bqp_ctx* c, prop_fn kern, int G, int64_t n, int* blocks) {
   return BQP_OK;
 }
 
+// The team kernel (bqp_team.hpp) takes FP64 Cowell and Encke calls that ask
+// for it (teams_per_block 1, 2 or 4 with an automatic group width) on fields
+// of degree >= PIPE_MIN_N: one 16-wave workgroup per CU carrying that many
+// groups.  Other calls take the fixed-G kernel.
+int shape_for(bqp_ctx* c, int kprec, int64_t n, LaunchShape* out) {
+  const bqp_launch& L = c->launch;
+  const int64_t groups = (n + 63) / 64;
+  *out = LaunchShape{};
+  const bool team = L.waves_per_group == 0 && L.teams_per_block > 0 && (kprec == 0 || kprec == 4) &&
+                    c->nmax >= PIPE_MIN_N;
+  if (team) {
+    const int nt = L.teams_per_block;
+    out->kern = (kprec == 4) ? pick_team_encke(c->cfg.method) : pick_team_fp64(c->cfg.method);
+    if (!out->kern) return fail(c, BQP_E_INVALID, "unknown method");
+    const int64_t need = (groups + nt - 1) / nt;
+    int64_t b = L.blocks > 0 ? L.blocks : std::min<int64_t>(need, c->cus);
+    b = std::max<int64_t>(1, std::min<int64_t>(b, need));
+    out->threads = 64 * TEAM_WAVES;
+    out->blocks = (int)b;
+    out->teams = nt;
+    out->queue0 = (unsigned)std::min<int64_t>(n, 64LL * nt * b);
+    return BQP_OK;
+  }
+  out->G = L.waves_per_group > 0 ? L.waves_per_group : default_g(c, n);
+  out->kern = pick_kernel(c->cfg.method, kprec, out->G);
+  if (!out->kern) return fail(c, BQP_E_INVALID, "waves_per_group must be 1, 2, 4, 8 or 16");
+  out->threads = 64 * out->G;
+  const int rc = grid_for(c, out->kern, out->G, n, &out->blocks);
+  if (rc) return rc;
+  out->queue0 = queue_start(n, out->blocks);
+  return BQP_OK;
+}
+
 // Checked surrogate propagation with the first check made up front: the
 // rows whose initial state fails it run the plain FP64 kernel (the same
 // arithmetic as the surrogate kernel's exact path, so the same results), the
 // others the surrogate kernel.  Both groups keep the longest-first order.
-int run_surrogate_partitioned(bqp_ctx* c, PropParams p, int64_t n, int kprec, int G, prop_fn sur_kern,
-                              bqp_stats* stats) {
-  const bqp_config& cfg = c->cfg;
-  prop_fn exact_kern = pick_kernel(cfg.method, 0, G);
-  if (!exact_kern) return fail(c, BQP_E_INVALID, "waves_per_group must be 1, 2, 4, 8 or 16");
+int run_surrogate_partitioned(bqp_ctx* c, PropParams p, int64_t n, int kprec, bqp_stats* stats) {
   int rc = ensure(c, &c->d_part, &c->part_bytes, (size_t)(3 * n + 64) * sizeof(int));
   if (rc) return rc;
   int* flags = (int*)c->d_part;
@@ -457,9 +497,10 @@ int run_surrogate_partitioned(bqp_ctx* c, PropParams p, int64_t n, int kprec, in
   int cnt[2] = {0, 0};
   HIP_TRY(c, hipMemcpyAsync(cnt, counts, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  int be = 1, bs = 1;
-  if (cnt[0] > 0 && (rc = grid_for(c, exact_kern, G, cnt[0], &be))) return rc;
-  if (cnt[1] > 0 && (rc = grid_for(c, sur_kern, G, cnt[1], &bs))) return rc;
+  // the failing rows take the kernel the plain FP64 form would take for them
+  LaunchShape se, ss;
+  if (cnt[0] > 0 && (rc = shape_for(c, 0, cnt[0], &se))) return rc;
+  if (cnt[1] > 0 && (rc = shape_for(c, kprec, cnt[1], &ss))) return rc;
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, p.sur_totals, 5);
   HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
   if (cnt[0] > 0) {
@@ -468,16 +509,18 @@ int run_surrogate_partitioned(bqp_ctx* c, PropParams p, int64_t n, int kprec, in
     q.n = cnt[0];
     q.sur_totals = nullptr;
     q.deal_cus = c->cus;
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue, queue_start(cnt[0], be));
-    hipLaunchKernelGGL(exact_kern, dim3(be), dim3(64 * G), 0, c->stream, q);
+    q.teams = se.teams;
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue, se.queue0);
+    hipLaunchKernelGGL(se.kern, dim3(se.blocks), dim3(se.threads), 0, c->stream, q);
   }
   if (cnt[1] > 0) {
     PropParams q = p;
     q.order = sur_rows;
     q.n = cnt[1];
     q.deal_cus = c->cus;
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue, queue_start(cnt[1], bs));
-    hipLaunchKernelGGL(sur_kern, dim3(bs), dim3(64 * G), 0, c->stream, q);
+    q.teams = ss.teams;
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, c->stream, c->d_queue, ss.queue0);
+    hipLaunchKernelGGL(ss.kern, dim3(ss.blocks), dim3(ss.threads), 0, c->stream, q);
   }
   HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, c->d_totals, 5);
@@ -545,9 +588,8 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats, const
     kprec = s->packed_fmt == BQP_MLP_FP8 ? 3 : 2;
   }
   if (cfg.formulation == BQP_FORMULATION_ENCKE) kprec = 4;
-  const int G = c->launch.waves_per_group > 0 ? c->launch.waves_per_group : default_g(c, n);
-  prop_fn kern = pick_kernel(cfg.method, kprec, G);
-  if (!kern) return fail(c, BQP_E_INVALID, "waves_per_group must be 1, 2, 4, 8 or 16");
+  if (c->launch.waves_per_group != 0 && !pick_kernel(cfg.method, kprec, c->launch.waves_per_group))
+    return fail(c, BQP_E_INVALID, "waves_per_group must be 1, 2, 4, 8 or 16");
   if (cfg.precision == BQP_PRECISION_SPLIT && !c->d_tabf)
     return fail(c, BQP_E_NO_FIELD, "split precision needs the FP32 table (reload the field)");
 
@@ -605,17 +647,13 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats, const
   }
 
   if (sur && c->mlp->check_every > 0 && !c->launch.use_graph && !fixed)
-    return run_surrogate_partitioned(c, p, n, kprec, G, kern, stats);
+    return run_surrogate_partitioned(c, p, n, kprec, stats);
 
-  const int64_t groups = (n + 63) / 64;
-  int blocks = c->launch.blocks;
-  if (blocks <= 0) {
-    int per_cu = 0;
-    HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kern, 64 * G, 0));
-    if (per_cu < 1) per_cu = 1;
-    blocks = (int)std::min<int64_t>(groups, (int64_t)per_cu * c->cus);
+  LaunchShape ls;
+  {
+    const int rc = shape_for(c, kprec, n, &ls);
+    if (rc) return rc;
   }
-  blocks = (int)std::max<int64_t>(1, std::min<int64_t>(blocks, groups));
 
   if (c->launch.use_graph && !fixed) {
     GraphKey key;
@@ -623,8 +661,9 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats, const
     key.p = p;
     key.n = n;
     key.cfg = cfg;
-    key.G = G;
-    key.blocks = blocks;
+    key.G = ls.G;
+    key.teams = ls.teams;
+    key.blocks = ls.blocks;
     key.sort = (sort ? 8 : 0) | c->launch.sort_by_cost;  // the mode decides the cost-model kernels
     key.tab = c->d_tab;
     key.sort_buf = sort ? c->d_sort : nullptr;
@@ -635,7 +674,7 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats, const
     if (!c->graph_exec || !(key == *c->graph_key)) {
       bqp_drop_graph(c);
       HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_propagation(c, p, n, kern, G, blocks, sort, false, sc);
+      enqueue_propagation(c, p, n, ls, sort, false, sc);
       hipGraph_t graph = nullptr;
       const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
       if (ce != hipSuccess || !graph) return fail(c, BQP_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(ce));
@@ -652,7 +691,7 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats, const
     HIP_TRY(c, hipGraphLaunch(c->graph_exec, c->stream));
     HIP_TRY(c, hipEventRecord(c->ev1, c->stream));
   } else {
-    enqueue_propagation(c, p, n, kern, G, blocks, sort, true, sc);
+    enqueue_propagation(c, p, n, ls, sort, true, sc);
   }
   HIP_TRY(c, hipGetLastError());
   HIP_TRY(c, hipStreamSynchronize(c->stream));
@@ -822,6 +861,8 @@ int32_t bqp_set_launch(bqp_ctx* c, const bqp_launch* l) {
   if (l->blocks < 0) return fail(c, BQP_E_INVALID, "blocks must be >= 0");
   if (l->use_graph != 0 && l->use_graph != 1) return fail(c, BQP_E_INVALID, "use_graph must be 0 or 1");
   if (l->sort_by_cost < 0 || l->sort_by_cost > 2) return fail(c, BQP_E_INVALID, "sort_by_cost must be 0, 1 or 2");
+  if (l->teams_per_block != 0 && l->teams_per_block != 1 && l->teams_per_block != 2 && l->teams_per_block != 4)
+    return fail(c, BQP_E_INVALID, "teams_per_block must be 0 (the fixed-G kernel), 1, 2 or 4");
   (void)hipStreamSynchronize(c->stream);
   bqp_drop_graph(c);
   c->launch = *l;

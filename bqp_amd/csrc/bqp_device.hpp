@@ -839,6 +839,214 @@ __device__ __forceinline__ void grav_split(const FieldView& f, int g, double x, 
   grav_split_ab<G>(f, g, x, y, z, ax, ay, az);
 }
 
+// ------------------------------------------------------ gravity in units
+// The team kernel splits one evaluation's column sweep over a varying number
+// of waves.  So that its result does not depend on that number, the columns
+// are cut into GRAV_UNITS fixed units and every unit is summed the same way
+// whichever wave computes it:
+//   * unit u holds the columns u + 32k (path A) and 31 - u + 32k (path B),
+//     k = 0, 1, ... (the back-and-forth deal over 32 columns, so every unit
+//     carries about the same number of entries); column 0, the zonal one,
+//     opens path A of unit 0;
+//   * a path starts from the sectoral value vs0 z^j of its first column,
+//     with z^j formed by square-and-multiply over the bits of j from the
+//     lowest (always the same products), and steps to the next column of the
+//     path by one complex multiply with z^32;
+//   * each column's terms go through the same pipelined loop as
+//     grav_fp64_pipelined; a unit's six running sums fold into (px, py, pz);
+//   * units are combined by the fixed pairwise tree over 0..15: a wave that
+//     owns an aligned block of 2^m units returns that block's subtree sum,
+//     and the caller combines the blocks further up the same tree.
+constexpr int GRAV_UNITS = 16;
+
+// (re, im) = z^k, 0 <= k < 64, by square-and-multiply from the lowest bit.
+__device__ __forceinline__ void cpow_bits(double zr, double zi, int k, double& re, double& im) {
+  re = 1.0;
+  im = 0.0;
+  double br = zr, bi = zi;
+  for (int b = 0; b < 6 && (k >> b) != 0; ++b) {
+    if ((k >> b) & 1) cmul(re, im, br, bi);
+    if ((k >> (b + 1)) != 0) cmul(br, bi, br, bi);
+  }
+}
+
+// Terms of column j >= 1 with sectoral value (vs, ws): the pipelined entry
+// loop of grav_fp64_pipelined (reads at most two entries past the column).
+#define BQP_COL_PIPE(EC, KK, VS, WS)                        \
+  {                                                         \
+    int kk = (KK);                                          \
+    BQP_WAIT_SMEM();                                        \
+    BQP_SCHED_FENCE();                                      \
+    const GravEntryL es = (EC)[0];                          \
+    GravEntryL a0 = (EC)[1], a1 = (EC)[2];                  \
+    BQP_SCHED_FENCE();                                      \
+    BQP_WAIT_SMEM();                                        \
+    BQP_SCHED_FENCE();                                      \
+    double v1 = (VS), w1 = (WS), v2 = 0.0, w2 = 0.0;        \
+    BQP_SECT_TERM(es, (VS), (WS));                          \
+    CEntryL r = (EC) + 3;                                   \
+    if (kk >= 2) {                                          \
+      kk -= 2;                                              \
+      while (kk >= 4) {                                     \
+        BQP_WAIT_SMEM();                                    \
+        BQP_SCHED_FENCE();                                  \
+        const GravEntryL b0 = r[0], b1 = r[1];              \
+        BQP_SCHED_FENCE();                                  \
+        BQP_COL_STEP(a0, v2, v1, w2, w1);                   \
+        BQP_COL_STEP(a1, v1, v2, w1, w2);                   \
+        BQP_WAIT_SMEM();                                    \
+        BQP_SCHED_FENCE();                                  \
+        a0 = r[2];                                          \
+        a1 = r[3];                                          \
+        BQP_SCHED_FENCE();                                  \
+        BQP_COL_STEP(b0, v2, v1, w2, w1);                   \
+        BQP_COL_STEP(b1, v1, v2, w1, w2);                   \
+        r += 4;                                             \
+        kk -= 4;                                            \
+      }                                                     \
+      BQP_WAIT_SMEM();                                      \
+      BQP_SCHED_FENCE();                                    \
+      GravEntryL t0, t1, t2;                                \
+      if (kk >= 1) t0 = r[0];                               \
+      if (kk >= 2) t1 = r[1];                               \
+      if (kk >= 3) t2 = r[2];                               \
+      BQP_SCHED_FENCE();                                    \
+      BQP_COL_STEP(a0, v2, v1, w2, w1);                     \
+      BQP_COL_STEP(a1, v1, v2, w1, w2);                     \
+      BQP_WAIT_SMEM();                                      \
+      BQP_SCHED_FENCE();                                    \
+      if (kk >= 1) BQP_COL_STEP(t0, v2, v1, w2, w1);        \
+      if (kk >= 2) BQP_COL_STEP(t1, v1, v2, w1, w2);        \
+      if (kk >= 3) BQP_COL_STEP(t2, v2, v1, w2, w1);        \
+    } else if (kk == 1) {                                   \
+      BQP_COL_STEP(a0, v2, v1, w2, w1);                     \
+    }                                                       \
+  }
+
+// Column 0 (zonal, W = 0) in 4-entry batches with the next batch in flight
+// (grav_fp64_pipelined's column 0); needs top >= PIPE_MIN_N + 1 entries.
+__device__ __forceinline__ void grav_col0_pipe(CEntryL e, int top, double vs0, double zt, double rho, double& sx,
+                                               double& sy, double& sz) {
+  double v1 = vs0, v2 = 0.0;
+  CEntryL r = e;
+  int k = top;
+  BQP_WAIT_SMEM();
+  BQP_SCHED_FENCE();
+  GravEntryL a0 = r[0], a1 = r[1], a2 = r[2], a3 = r[3];
+  BQP_SCHED_FENCE();
+  while (k >= 8) {
+    BQP_WAIT_SMEM();
+    BQP_SCHED_FENCE();
+    const GravEntryL b0 = r[4], b1 = r[5], b2 = r[6], b3 = r[7];
+    BQP_SCHED_FENCE();
+    BQP_COL0_STEP(a0, v2, v1);
+    BQP_COL0_STEP(a1, v1, v2);
+    BQP_COL0_STEP(a2, v2, v1);
+    BQP_COL0_STEP(a3, v1, v2);
+    BQP_WAIT_SMEM();
+    BQP_SCHED_FENCE();
+    a0 = r[8];
+    a1 = r[9];
+    a2 = r[10];
+    a3 = r[11];
+    BQP_SCHED_FENCE();
+    BQP_COL0_STEP(b0, v2, v1);
+    BQP_COL0_STEP(b1, v1, v2);
+    BQP_COL0_STEP(b2, v2, v1);
+    BQP_COL0_STEP(b3, v1, v2);
+    r += 8;
+    k -= 8;
+  }
+  BQP_WAIT_SMEM();
+  BQP_SCHED_FENCE();
+  GravEntryL b0, b1, b2;
+  if (k >= 5) b0 = r[4];
+  if (k >= 6) b1 = r[5];
+  if (k >= 7) b2 = r[6];
+  BQP_SCHED_FENCE();
+  if (k >= 1) BQP_COL0_STEP(a0, v2, v1);
+  if (k >= 2) BQP_COL0_STEP(a1, v1, v2);
+  if (k >= 3) BQP_COL0_STEP(a2, v2, v1);
+  if (k >= 4) BQP_COL0_STEP(a3, v1, v2);
+  BQP_WAIT_SMEM();
+  BQP_SCHED_FENCE();
+  if (k >= 5) BQP_COL0_STEP(b0, v2, v1);
+  if (k >= 6) BQP_COL0_STEP(b1, v1, v2);
+  if (k >= 7) BQP_COL0_STEP(b2, v2, v1);
+}
+
+// Partial acceleration (body frame) of unit u at a body-fixed position whose
+// scaled coordinates are given (xt = x R / r^2, ..., rho = R^2 / r^2,
+// vs0 = R / r; (z32r, z32i) = (xt + i yt)^32).  FP64 rescaled table, degree
+// >= PIPE_MIN_N.
+__device__ __forceinline__ void grav_unit(const FieldView& f, int u, double xt, double yt, double zt, double rho,
+                                          double vs0, double z32r, double z32i, double& px, double& py,
+                                          double& pz) {
+  double sx = 0.0, sy = 0.0, sz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
+  const int top = f.nmax + 1;
+  CEntryL e = (CEntryL)f.tabl;
+  CEntryL e1 = e + top;  // column 1
+  for (int path = 0; path < 2; ++path) {
+    int j = path == 0 ? u : 2 * GRAV_UNITS - 1 - u;
+    double vs, ws;
+    if (j == 0) {  // path A of unit 0 opens with the zonal column
+      grav_col0_pipe(e, top, vs0, zt, rho, sx, sy, sz);
+      j = 2 * GRAV_UNITS;
+      vs = vs0 * z32r;
+      ws = vs0 * z32i;
+    } else {
+      double pr, pi;
+      cpow_bits(xt, yt, j, pr, pi);
+      vs = vs0 * pr;
+      ws = vs0 * pi;
+    }
+    for (; j <= f.jmax; j += 2 * GRAV_UNITS) {
+      const CEntryL ec = e1 + (col_offset(f.nmax, j) - top);
+      BQP_COL_PIPE(ec, top - j, vs, ws);
+      cmul(vs, ws, z32r, z32i);
+    }
+  }
+  px = sx + tx;
+  py = sy + ty;
+  pz = sz + tz;
+}
+
+// A position prepared for unit sums: the inertial position turned into the
+// body frame by the angle whose cosine and sine are (c, s), its scaled
+// coordinates and z^32.
+struct UnitPoint {
+  double xt, yt, zt, rho, vs0, z32r, z32i, c, s;
+};
+
+__device__ __forceinline__ UnitPoint unit_point(const FieldView& f, double xi, double yi, double zi, double c,
+                                                double s) {
+  UnitPoint q;
+  const double x = fma(c, xi, s * yi), y = fma(c, yi, -(s * xi)), z = zi;
+  const double R = f.radius;
+  const double r2 = fma(x, x, fma(y, y, z * z));
+  const double rr = 1.0 / r2;
+  q.xt = x * R * rr;
+  q.yt = y * R * rr;
+  q.zt = z * R * rr;
+  q.rho = R * R * rr;
+  q.vs0 = R / sqrt(r2);
+  cpow_bits(q.xt, q.yt, 2 * GRAV_UNITS, q.z32r, q.z32i);
+  q.c = c;
+  q.s = s;
+  return q;
+}
+
+// Unit u's partial turned back to the inertial frame: the same value
+// whichever wave computes it.
+__device__ __forceinline__ void grav_unit_inertial(const FieldView& f, const UnitPoint& q, int u, double& px,
+                                                   double& py, double& pz) {
+  double gx, gy, gz;
+  grav_unit(f, u, q.xt, q.yt, q.zt, q.rho, q.vs0, q.z32r, q.z32i, gx, gy, gz);
+  px = fma(q.c, gx, -(q.s * gy));
+  py = fma(q.s, gx, q.c * gy);
+  pz = gz;
+}
+
 // ------------------------------------------------------------- atmosphere
 
 // Vallado (2013) Table 8-4: band base altitude (km), nominal density

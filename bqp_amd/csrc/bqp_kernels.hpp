@@ -142,6 +142,7 @@ struct PropParams {
   unsigned int* queue;   // next queue position; starts after the first round
   int n;
   int deal_cus;          // CUs the first round of groups is dealt over (0: block order)
+  int teams;             // team kernel: trajectory groups per workgroup (1, 2 or 4)
   int m_eval;
   double t0c;
   FieldView field;

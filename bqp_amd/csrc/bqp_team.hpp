@@ -1,0 +1,661 @@
+// Team kernel: the persistent batched propagation with the gravity work of
+// one CU shared out among its trajectory groups as they finish (gfx950).
+//
+// One workgroup of TEAM_WAVES = 16 waves per CU (four per SIMD) carries up to
+// TEAM_MAX = 4 groups of 64 trajectories ("teams"), one trajectory per lane.
+// Wave t < NT is the leader of team t and alone holds that team's integrator
+// state; every wave, leaders included, computes gravity.  Each loop
+// iteration is one right-hand-side evaluation for every live team:
+//   A  each leader starts a step if its team has none in flight (refilling
+//      rows), then writes the team's next stage state, frame angle and
+//      Cd*A/m to LDS, and its liveness;
+//   -- barrier --
+//   B  every wave takes work items from one LDS counter until none are
+//      left: an item is one gravity unit of a live team (see grav_unit;
+//      two or fewer live teams) or an aligned pair of units (three or four),
+//      so 16 waves always find at least 16 items.  The wave turns the team's
+//      stage position into the body frame (sincos of the team's angle),
+//      sums the item's units, turns each back to the inertial frame and
+//      writes the item's sum to its LDS slot; the wave that takes a team's
+//      first item also computes the team's drag;
+//   -- barrier --
+//   C  each leader combines its team's slots along the fixed pairwise tree
+//      over the 16 units, adds the drag (and the Encke terms), stores the
+//      stage and, after the last stage, completes the step (solution, error
+//      norm, controller, row results).
+// When a team runs out of rows its items leave the pool, so the CU's waves
+// all work on the groups still running until the last row of the CU ends.
+// Every unit is computed the same way whichever wave takes it, and the tree
+// is fixed, so a team's accelerations, and the results, do not depend on the
+// number of live teams, the grid, the hand-out order or the batch a row is
+// part of.  The price is lockstep: each evaluation waits for the last item
+// and for the leaders' serial work, which on the 65,536-row headline batch
+// costs more than the tail it removes, so the fixed-G kernel stays the
+// default and this one runs on request (bqp_launch.teams_per_block).
+//
+// The per-row integrator is the one of propagate_kernel (bqp_kernels.hpp):
+// rk_step and the controller restated from LY/src/odesolve/solver.rs:
+// 445-529, 1099-1925, 1965-2173 and LY/src/odesolve/lightyear_compat.rs:
+// 100-600, the Encke segment driver of LY/src/integrator.rs:341-402,
+// 3418-3563; it is run here by the leader alone as a per-evaluation state
+// machine (the stage index s_cur).
+#pragma once
+
+#include "bqp_kernels.hpp"
+
+namespace bqp {
+
+constexpr int TEAM_WAVES = 16;
+constexpr int TEAM_MAX = 4;
+// LDS slots for the iteration's gravity work items: the 16 units of each of
+// up to TEAM_PAIRS_ABOVE live teams, else 8 aligned unit pairs per team
+constexpr int TEAM_PAIRS_ABOVE = 2;
+constexpr int TEAM_SLOTS = 2 * GRAV_UNITS;
+
+// First group (in hand-out positions) of team t of block b: the teams of the
+// blocks are dealt in rounds, round t over blocks 0..C-1 in order, odd rounds
+// in reverse, so each block's teams take ranks b, 2C-1-b, 2C+b, 4C-1-b of the
+// longest-first order and the CUs carry nearly equal work.
+__device__ __forceinline__ unsigned team_first_group(unsigned b, unsigned t, unsigned C) {
+  return t * C + ((t & 1u) ? (C - 1u - b) : b);
+}
+
+template <int METHOD, int PREC>
+__global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const PropParams p) {
+  using MT = MethodTraits<METHOD>;
+  constexpr int S = MT::S;
+  constexpr bool ENCKE = (PREC == 4);
+  static_assert(PREC == 0 || PREC == 4, "team kernel: plain FP64 Cowell or Encke");
+  const bqp_tableau& TB = c_tab[METHOD];
+  const SecondOrderTableau& SO = c_so[METHOD];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int NT = p.teams;  // 1, 2 or 4 teams per block
+  const bool leader = w < NT;
+  const int team = w;
+
+  __shared__ double sA[TEAM_MAX][S][3][64];   // stage accelerations of each team
+  __shared__ double sYt[TEAM_MAX][6][64];     // inertial stage states (total state for Encke)
+  __shared__ double sTh[TEAM_MAX][64];        // body-frame angle at the stage time
+  __shared__ double sBc[TEAM_MAX][64];        // Cd*A/m of the lanes that store the stage (else 0)
+  __shared__ double sDrag[TEAM_MAX][3][64];   // drag accelerations of the stage
+  __shared__ double sSlot[TEAM_SLOTS][3][64]; // gravity subtree sums of the iteration's work items
+  __shared__ int sAlive[TEAM_MAX];
+  __shared__ int sNext;                       // next work item of the iteration
+
+  const bool compat = (p.control == CONTROL_COMPAT);
+  const bool sampled = (p.t_eval != nullptr);
+  const double q = compat ? 5.0 : (double)(TB.order_err + 1);
+  const double inv_q = 1.0 / q;
+
+  // ---- leader state (the team's 64 rows)
+  bool alive = leader, in_step = false, exhausted = false, first_fill = true;
+  bool active = false;
+  int row = 0, steps = 0, evals = 0, rejects = 0, rej_run = 0, sidx = 0;
+  double t = 0.0, tseg = 0.0, tend = 0.0, h = 0.0, tcomp = 0.0, eprev = 0.0, bc = 0.0;
+  double hmin = 0.0, hmax = 0.0, eps = p.eps;
+  bool have_prev = false, just_rej = false, have_k1 = false;
+  double y[6] = {0, 0, 0, 0, 0, 0};
+  EnckeLane el{};
+  // step attempt and stage in flight
+  int s_cur = 0;
+  bool stepping = false, needk = false, lands = false, st = false;
+  double hs = 0.0;
+  double bst[3] = {0, 0, 0}, ysr[3] = {0, 0, 0};
+  const bool drag_on = p.model.drag != DRAG_NONE;
+
+  auto finish = [&](int code) {
+    if (code != ROW_OK) {
+      if (sampled) {
+        for (int m = sidx; m < p.m_eval; ++m)
+          for (int i = 0; i < 6; ++i) p.ysamp[((size_t)row * p.m_eval + m) * 6 + i] = __builtin_nan("");
+      } else {
+        for (int i = 0; i < 6; ++i) p.yf[(size_t)row * 6 + i] = __builtin_nan("");
+      }
+    } else if (!sampled) {
+      for (int i = 0; i < 6; ++i) p.yf[(size_t)row * 6 + i] = y[i];
+    }
+    if (p.status) p.status[row] = code;
+    if (p.steps) p.steps[row] = steps;
+    if (p.evals) p.evals[row] = evals;
+    if (p.rejects) p.rejects[row] = rejects;
+    active = false;
+  };
+
+  auto start_controller = [&](double span) {
+    if (compat) {
+      eps = fmax(p.eps, 1e-12);
+      hmin = 1e-10;
+      hmax = (p.h_max > 0.0 && isfinite(p.h_max)) ? fmax(p.h_max, hmin) : span;
+      h = clampd(fabs(span / 100.0), 1e-3, fmax(hmax, 1e-3));
+    } else {
+      hmax = (p.h_max > 0.0 && isfinite(p.h_max)) ? p.h_max : span;
+      hmin = (p.h_min > 0.0) ? p.h_min : 1e-12;
+      if (hmax < hmin) hmax = hmin;
+      const double h0 = (p.h0 > 0.0) ? p.h0 : ((span <= 60.0) ? span / 2.0 : span / 100.0);
+      h = clampd(h0, hmin, hmax);
+    }
+  };
+
+  auto start_segment = [&](const double* tot, double carry) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      el.base[i] = tot[i];
+      y[i] = 0.0;
+    }
+    el.t0 = t;
+    el.x = __builtin_nan("");
+    el.seg_end = (tend - t > ENCKE_SEGMENT_S) ? t + ENCKE_SEGMENT_S : tend;
+    start_controller(el.seg_end - t);
+    if (carry > 0.0 && !compat && !(p.h_max > 0.0 && p.h_max <= HCARRY_MIN_CAP_S)) h = clampd(carry, hmin, hmax);
+    have_prev = just_rej = have_k1 = false;
+    eprev = 0.0;
+    rej_run = 0;
+    tcomp = 0.0;
+  };
+
+  auto total_state = [&](double* tot) {
+    if constexpr (ENCKE) {
+      double b[6];
+      two_body(p.mu, el.base, t - el.t0, b, el.x);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) tot[i] = b[i] + y[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) tot[i] = y[i];
+    }
+  };
+
+  // Takes rows for free lanes (the dealt first group, then the queue) and
+  // sets each new row up; returns with the lanes' new rows started.
+  auto refill = [&]() {
+    const unsigned long long need = wave_ballot(!active);
+    if (need == 0ull || exhausted) return;
+    const int cnt = __popcll(need);
+    unsigned int base;
+    const bool dealt = first_fill;
+    if (first_fill) {
+      base = 64u * team_first_group(blockIdx.x, (unsigned)team, gridDim.x);
+      first_fill = false;
+    } else {
+      unsigned int b = 0;
+      const int first = __ffsll((long long)need) - 1;
+      if (lane == first) b = atomicAdd(p.queue, (unsigned)cnt);
+      base = __shfl(b, first);
+    }
+    if (dealt)
+      exhausted = 64ull * (unsigned long long)NT * gridDim.x >= (unsigned long long)p.n;
+    else if (base + (unsigned)cnt >= (unsigned)p.n)
+      exhausted = true;
+    if (active) return;
+    const unsigned pos = base + (unsigned)lane_rank(need);
+    if (pos >= (unsigned)p.n) return;
+    row = p.order ? p.order[pos] : (int)pos;
+    active = true;
+    steps = evals = rejects = rej_run = 0;
+    sidx = 0;
+    tcomp = 0.0;
+    eprev = 0.0;
+    have_prev = just_rej = have_k1 = false;
+    bc = p.bc ? p.bc[row] : 0.0;
+    bool ok = isfinite(bc);
+    for (int i = 0; i < 6; ++i) {
+      y[i] = p.y0[(size_t)row * 6 + i];
+      ok = ok && isfinite(y[i]);
+    }
+    t = sampled ? p.t0c : p.t0[row];
+    tend = sampled ? p.t_eval[p.m_eval - 1] : p.tf[row];
+    ok = ok && isfinite(t) && isfinite(tend) && (tend >= t);
+    if (!ok) {
+      finish(ROW_INVALID_INPUT);
+    } else if (p.model.r_min > 0.0 && !above_ground(p.model, y[0], y[1], y[2])) {
+      finish(ROW_BELOW_MIN_RADIUS);
+    } else {
+      if (sampled) {
+        while (sidx < p.m_eval && p.t_eval[sidx] == t) {
+          for (int i = 0; i < 6; ++i) p.ysamp[((size_t)row * p.m_eval + sidx) * 6 + i] = y[i];
+          ++sidx;
+        }
+      }
+      const double span = tend - t;
+      if (span == 0.0 || (sampled && sidx >= p.m_eval)) {
+        finish(ROW_OK);
+      } else if constexpr (ENCKE) {
+        double tot[6], probe[6], x0 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) tot[i] = y[i];
+        if (!two_body(p.mu, tot, 0.0, probe, x0)) {
+          finish(ROW_INVALID_INPUT);
+        } else {
+          start_segment(tot, 0.0);
+          tseg = sampled ? fmin(p.t_eval[sidx], el.seg_end) : el.seg_end;
+        }
+      } else {
+        tseg = sampled ? p.t_eval[sidx] : tend;
+        start_controller(span);
+      }
+    }
+  };
+
+  // Completes a step attempt once all its stages are stored (rk_step's sums,
+  // the error norm and the controller), writing rows that end.
+  auto complete_step = [&]() {
+    double yn[6], err[6], comp[6], err3[6], comp3[6];
+    {
+      const double hb = hs * SO.bsum, he = hs * SO.esum, he3 = hs * SO.e3sum;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        yn[i] = fma(y[3 + i], hb, y[i]);
+        yn[3 + i] = y[3 + i];
+        err[i] = he * y[3 + i];
+        err[3 + i] = 0.0;
+        err3[i] = MT::E3 ? he3 * y[3 + i] : 0.0;
+        err3[3 + i] = 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < 6; ++i) comp[i] = comp3[i] = 0.0;
+    }
+    const double hh = hs * hs;
+    auto kahan = [](double& sum, double& c, double term) {
+      const double yk = term - c;
+      const double acc = sum + yk;
+      c = (acc - sum) - yk;
+      sum = acc;
+    };
+#pragma unroll
+    for (int j = 0; j < S; ++j) {
+      const double bj = TB.b[j], ej = TB.e[j], baj = SO.ba[j], eaj = SO.ea[j];
+      if (bj != 0.0) {
+        const double hbj = hs * bj;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) yn[3 + i] = fma(sA[team][j][i][lane], hbj, yn[3 + i]);
+      }
+      if (baj != 0.0) {
+        const double hba = hh * baj;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) yn[i] = fma(sA[team][j][i][lane], hba, yn[i]);
+      }
+      if (ej != 0.0) {
+        const double hej = hs * ej;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) kahan(err[3 + i], comp[3 + i], hej * sA[team][j][i][lane]);
+      }
+      if (eaj != 0.0) {
+        const double hea = hh * eaj;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) kahan(err[i], comp[i], hea * sA[team][j][i][lane]);
+      }
+      if (MT::E3) {
+        const double e3j = TB.e3[j], e3aj = SO.e3a[j];
+        if (e3j != 0.0) {
+          const double h3 = hs * e3j;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) kahan(err3[3 + i], comp3[3 + i], h3 * sA[team][j][i][lane]);
+        }
+        if (e3aj != 0.0) {
+          const double h3a = hh * e3aj;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) kahan(err3[i], comp3[i], h3a * sA[team][j][i][lane]);
+        }
+      }
+    }
+    if (!stepping) return;
+    bool finite = true;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) finite = finite && isfinite(yn[i]) && isfinite(err[i]);
+    double en = 0.0, thresh = 1.0;
+    if (finite) {
+      const bool blend3 = MT::E3 && (p.control == CONTROL_SCALED ||
+                                     (p.control == CONTROL_ABSOLUTE && eps >= 1e-6));
+      if (p.control == CONTROL_SCALED) {
+        double s5 = 0.0, s3 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          const double atol = (i < 3) ? p.atol_pos : p.atol_vel;
+          const double sc = atol + p.rtol * fmax(fabs(y[i]), fabs(yn[i]));
+          const double den = (sc > 0.0) ? sc : atol;
+          const double e5 = err[i] / den;
+          s5 = fma(e5, e5, s5);
+          if (blend3) {
+            const double e3 = err3[i] / den;
+            s3 = fma(e3, e3, s3);
+          }
+        }
+        const double rms5 = sqrt(s5 / 6.0);
+        if (blend3) {
+          const double rms3 = sqrt(s3 / 6.0);
+          en = sqrt(fma(rms5, rms5, 0.01 * rms3 * rms3));
+        } else {
+          en = rms5;
+        }
+      } else {
+        double m5 = 0.0, m3 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          m5 = fmax(m5, fabs(err[i]));
+          if (blend3 && !compat) m3 = fmax(m3, fabs(err3[i]));
+        }
+        en = (blend3 && !compat) ? sqrt(fma(m5, m5, 0.01 * m3 * m3)) : m5;
+        thresh = eps;
+      }
+      finite = isfinite(en);
+    }
+    if (!finite) {
+      ++rejects;
+      if (++rej_run > p.max_rejects) {
+        finish(ROW_NON_FINITE);
+      } else {
+        h = clampd(hs * 0.1, hmin, hmax);
+      }
+    } else if (en <= thresh || hs <= hmin) {
+      double tn;
+      if (lands) {
+        tn = tseg;
+      } else {
+        const double ky = hs - tcomp;
+        tn = t + ky;
+        tcomp = (tn - t) - ky;
+      }
+      ++steps;
+      rej_run = 0;
+      if (MT::FSAL) {
+        for (int i = 0; i < 3; ++i) sA[team][0][i][lane] = sA[team][S - 1][i][lane];
+        have_k1 = true;
+      } else {
+        have_k1 = false;
+      }
+      t = tn;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) y[i] = yn[i];
+      const double eff = (en > 0.0) ? en : thresh * 0.1;
+      double fac, maxg;
+      if (compat) {
+        fac = (have_prev && eprev > 0.0) ? 0.9 * pow(eps / eff, 0.7 / 5.0) * pow(eprev / eps, 0.4 / 5.0)
+                                         : 0.9 * pow(eps / eff, 0.2);
+        maxg = just_rej ? 2.0 : 5.0;
+      } else {
+        if (p.control == CONTROL_SCALED)
+          fac = (have_prev && eprev > 0.0) ? 0.9 * pow(1.0 / eff, 0.7 / q) * pow(eprev, 0.4 / q)
+                                           : 0.9 * pow(eff, -inv_q);
+        else
+          fac = (have_prev && eprev > 0.0)
+                    ? 0.9 * pow(eps / eff, 0.7 / q) * pow(eprev / eps, 0.4 / q)
+                    : 0.9 * pow(eps / eff, inv_q);
+        const double base = fmin(1.0 + 4.0 * (5.0 / q), 5.0);
+        maxg = just_rej ? fmin(base, 2.0) : base;
+      }
+      fac = clampd(fac, 0.2, maxg);
+      h = clampd(hs * fac, hmin, hmax);
+      eprev = en;
+      have_prev = true;
+      just_rej = false;
+      double rmin_tot[6];
+      if (p.model.r_min > 0.0) total_state(rmin_tot);
+      if (p.model.r_min > 0.0 && !above_ground(p.model, rmin_tot[0], rmin_tot[1], rmin_tot[2])) {
+        finish(ROW_BELOW_MIN_RADIUS);
+      } else if (ENCKE && t == tseg) {
+        double tot[6];
+        total_state(tot);
+        bool done = false;
+        if (sampled) {
+          if (sidx < p.m_eval && t == p.t_eval[sidx]) {
+            for (int i = 0; i < 6; ++i) p.ysamp[((size_t)row * p.m_eval + sidx) * 6 + i] = tot[i];
+            ++sidx;
+          }
+          done = (sidx >= p.m_eval);
+        } else {
+          done = (t == tend);
+        }
+        if (done) {
+#pragma unroll
+          for (int i = 0; i < 6; ++i) y[i] = tot[i];
+          finish(ROW_OK);
+        } else {
+          if (t == el.seg_end) start_segment(tot, h);
+          tseg = sampled ? fmin(p.t_eval[sidx], el.seg_end) : el.seg_end;
+          tcomp = 0.0;
+        }
+      } else if (t == tseg) {
+        if (sampled) {
+          for (int i = 0; i < 6; ++i) p.ysamp[((size_t)row * p.m_eval + sidx) * 6 + i] = y[i];
+          ++sidx;
+          if (sidx >= p.m_eval) {
+            finish(ROW_OK);
+          } else {
+            tseg = p.t_eval[sidx];
+            tcomp = 0.0;
+          }
+        } else {
+          finish(ROW_OK);
+        }
+      }
+    } else {
+      ++rejects;
+      if (++rej_run > p.max_rejects) {
+        finish(ROW_MAX_REJECTS);
+      } else {
+        double fac;
+        if (p.control == CONTROL_SCALED)
+          fac = 0.9 * pow(en, -inv_q);
+        else
+          fac = 0.9 * pow(eps / en, compat ? 0.2 : inv_q);
+        fac = clampd(fac, 0.1, 0.5);
+        h = clampd(hs * fac, hmin, hmax);
+        if (compat) {
+          eprev = en;
+          have_prev = true;
+        }
+        just_rej = true;
+      }
+    }
+  };
+
+  while (true) {
+    // ------------------------------------------------------------ A
+    if (leader) {
+      if (alive && !in_step) {
+        while (true) {
+          refill();
+          if (wave_ballot(active) == 0ull) {
+            if (exhausted) {
+              alive = false;
+              break;
+            }
+            continue;
+          }
+          stepping = active;
+          const double rem = tseg - t;
+          hs = (h < rem) ? h : rem;
+          lands = (hs == rem);
+          if (stepping) {
+            const int budget = compat ? steps + rejects : steps;
+            if (budget >= p.max_steps || steps + rejects >= 2 * p.max_steps) {
+              finish(ROW_MAX_STEPS);
+              stepping = false;
+            } else if (!compat && hs < hmin && !lands) {
+              finish(ROW_STEP_UNDERFLOW);
+              stepping = false;
+            }
+          }
+          if (wave_ballot(stepping) == 0ull) continue;
+          needk = stepping && !have_k1;
+          s_cur = wave_ballot(needk) != 0ull ? 0 : 1;
+          in_step = true;
+          break;
+        }
+      }
+      if (alive) {
+        const int s = s_cur;
+        st = (s == 0) ? needk : stepping;
+        // stage input (second-order form, as propagate_kernel)
+        double ys[6];
+        const double hh = hs * hs;
+        const double hc = hs * SO.csum[s];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          ys[i] = fma(y[3 + i], hc, y[i]);
+          ys[3 + i] = y[3 + i];
+        }
+#pragma unroll
+        for (int j = 0; j < S - 1; ++j) {
+          if (j < s) {
+            const double a = TB.a[s][j];
+            if (a != 0.0) {
+              const double sv = hs * a;
+#pragma unroll
+              for (int i = 0; i < 3; ++i) ys[3 + i] = fma(sA[team][j][i][lane], sv, ys[3 + i]);
+            }
+            const double a2 = SO.a2[s][j];
+            if (a2 != 0.0) {
+              const double sr = hh * a2;
+#pragma unroll
+              for (int i = 0; i < 3; ++i) ys[i] = fma(sA[team][j][i][lane], sr, ys[i]);
+            }
+          }
+        }
+        const double tst = t + TB.c[s] * hs;
+        if constexpr (ENCKE) {
+          double b6[6];
+          two_body(p.mu, el.base, tst - el.t0, b6, el.x);
+#pragma unroll
+          for (int i = 0; i < 6; ++i) sYt[team][i][lane] = b6[i] + ys[i];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            bst[i] = b6[i];
+            ysr[i] = ys[i];
+          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < 6; ++i) sYt[team][i][lane] = ys[i];
+        }
+        sTh[team][lane] = fma(p.model.omega, tst, p.model.theta0);
+        sBc[team][lane] = st ? bc : 0.0;
+      }
+      if (lane == 0) sAlive[team] = alive ? 1 : 0;
+    }
+    if (w == 0 && lane == 0) sNext = 0;
+    __syncthreads();
+    // ------------------------------------------------------------ B
+    // the live teams in index order; work items are the units (two or fewer
+    // live teams) or aligned unit pairs (three or four) of every live team
+    int live = 0, pos_of_team = -1;
+    int live_id[TEAM_MAX];
+#pragma unroll
+    for (int k = 0; k < TEAM_MAX; ++k) {
+      const int a = (k < NT) ? sAlive[k] : 0;
+      if (leader && k == team && a) pos_of_team = live;
+      live_id[k] = -1;
+      if (a) {
+#pragma unroll
+        for (int m = 0; m < TEAM_MAX; ++m)
+          if (m == live) live_id[m] = k;
+        ++live;
+      }
+    }
+    if (live == 0) break;
+    const bool pairs = live > TEAM_PAIRS_ABOVE;
+    const int per_team = pairs ? GRAV_UNITS / 2 : GRAV_UNITS;
+    const int items = live * per_team;
+    {
+      int cur = -1;
+      UnitPoint q{};
+      while (true) {
+        int it = 0;
+        if (lane == 0) it = atomicAdd(&sNext, 1);
+        it = __builtin_amdgcn_readfirstlane(__shfl(it, 0));
+        if (it >= items) break;
+        const int lp = it / per_team, sub = it - lp * per_team;
+        int k = -1;
+#pragma unroll
+        for (int m = 0; m < TEAM_MAX; ++m)
+          if (m == lp) k = live_id[m];
+        k = __builtin_amdgcn_readfirstlane(k);
+        if (k != cur) {  // this team's stage position, turned into the body frame
+          double sth, cth;
+          sincos(sTh[k][lane], &sth, &cth);
+          q = unit_point(p.field, sYt[k][0][lane], sYt[k][1][lane], sYt[k][2][lane], cth, sth);
+          cur = k;
+        }
+        double px, py, pz;
+        if (pairs) {
+          double bx, by, bz;
+          grav_unit_inertial(p.field, q, 2 * sub, px, py, pz);
+          grav_unit_inertial(p.field, q, 2 * sub + 1, bx, by, bz);
+          px += bx;
+          py += by;
+          pz += bz;
+        } else {
+          grav_unit_inertial(p.field, q, sub, px, py, pz);
+        }
+        sSlot[it][0][lane] = px;
+        sSlot[it][1][lane] = py;
+        sSlot[it][2][lane] = pz;
+        // the wave that takes a team's first item also computes its drag
+        // (LY/src/rhs.rs:1119-1163)
+        if (drag_on && sub == 0) {
+          const double b = sBc[k][lane];
+          double dx = 0.0, dy = 0.0, dz = 0.0;
+          if (b != 0.0) {
+            const double y0 = sYt[k][0][lane], y1 = sYt[k][1][lane], y2 = sYt[k][2][lane];
+            const double r = sqrt(fma(y0, y0, fma(y1, y1, y2 * y2)));
+            const double rho = density(p.model.drag, r - p.model.r_atm);
+            if (rho > 0.0) {
+              const double vrx = fma(p.model.omega, y1, sYt[k][3][lane]);
+              const double vry = fma(-p.model.omega, y0, sYt[k][4][lane]);
+              const double vrz = sYt[k][5][lane];
+              const double vr = sqrt(fma(vrx, vrx, fma(vry, vry, vrz * vrz)));
+              const double kk = -0.5 * b * rho * vr;
+              dx = kk * vrx;
+              dy = kk * vry;
+              dz = kk * vrz;
+            }
+          }
+          sDrag[k][0][lane] = dx;
+          sDrag[k][1][lane] = dy;
+          sDrag[k][2][lane] = dz;
+        }
+      }
+    }
+    __syncthreads();
+    // ------------------------------------------------------------ C
+    if (leader && alive) {
+      // the team's slots combined along the unit tree (in place): units in
+      // pairs first when the items were single units, then pairs up to the
+      // root
+      const int base = pos_of_team * per_team;
+      for (int stride = 1; stride < per_team; stride *= 2)
+        for (int r = 0; r + stride < per_team; r += 2 * stride)
+#pragma unroll
+          for (int i = 0; i < 3; ++i) sSlot[base + r][i][lane] += sSlot[base + r + stride][i][lane];
+      double ax = sSlot[base][0][lane], ay = sSlot[base][1][lane], az = sSlot[base][2][lane];
+      if (drag_on) {
+        ax += sDrag[team][0][lane];
+        ay += sDrag[team][1][lane];
+        az += sDrag[team][2][lane];
+      }
+      if constexpr (ENCKE) {
+        const double y0 = sYt[team][0][lane], y1 = sYt[team][1][lane], y2 = sYt[team][2][lane];
+        const double r2 = fma(y0, y0, fma(y1, y1, y2 * y2));
+        const double k = p.mu / (r2 * sqrt(r2));
+        ax = fma(k, y0, ax);
+        ay = fma(k, y1, ay);
+        az = fma(k, y2, az);
+        battin(p.mu, bst, ysr, ax, ay, az);
+      }
+      if (st) {
+        sA[team][s_cur][0][lane] = ax;
+        sA[team][s_cur][1][lane] = ay;
+        sA[team][s_cur][2][lane] = az;
+        ++evals;
+      }
+      if (s_cur == 0 && needk) have_k1 = true;
+      ++s_cur;
+      if (s_cur == S) {
+        complete_step();
+        in_step = false;
+      }
+    }
+  }
+}
+
+}  // namespace bqp

@@ -14,4 +14,13 @@ prop_fn pick_encke(int method, int G) {
     default: return nullptr;
   }
 }
+
+prop_fn pick_team_encke(int method) {
+  switch (method) {
+    case BQP_METHOD_DOPRI5: return propagate_team_kernel<BQP_METHOD_DOPRI5, 4>;
+    case BQP_METHOD_DOP853: return propagate_team_kernel<BQP_METHOD_DOP853, 4>;
+    case BQP_METHOD_RK87: return propagate_team_kernel<BQP_METHOD_RK87, 4>;
+    default: return nullptr;
+  }
+}
 }  // namespace bqp

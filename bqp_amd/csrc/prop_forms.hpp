@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include "bqp_kernels.hpp"
+#include "bqp_team.hpp"
 
 namespace bqp {
 
@@ -21,6 +22,9 @@ prop_fn pick_sur(int method, int prec, int G);  // prec 2 (BF16 network) or 3 (F
 prop_fn pick_sur_bf16(int method, int G);
 prop_fn pick_sur_fp8(int method, int G);
 prop_fn pick_encke(int method, int G);
+// team kernel (bqp_team.hpp) for a method, FP64 Cowell or Encke
+prop_fn pick_team_fp64(int method);
+prop_fn pick_team_encke(int method);
 
 hipError_t upload_fp64(const SecondOrderTableau* so, const double* sect, const float* sectf);
 hipError_t upload_split(const SecondOrderTableau* so, const double* sect, const float* sectf);

@@ -13,4 +13,13 @@ prop_fn pick_fp64(int method, int G) {
     default: return nullptr;
   }
 }
+
+prop_fn pick_team_fp64(int method) {
+  switch (method) {
+    case BQP_METHOD_DOPRI5: return propagate_team_kernel<BQP_METHOD_DOPRI5, 0>;
+    case BQP_METHOD_DOP853: return propagate_team_kernel<BQP_METHOD_DOP853, 0>;
+    case BQP_METHOD_RK87: return propagate_team_kernel<BQP_METHOD_RK87, 0>;
+    default: return nullptr;
+  }
+}
 }  // namespace bqp

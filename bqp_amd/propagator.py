@@ -145,15 +145,19 @@ class Propagator:
     def __exit__(self, *exc):
         self.close()
 
-    def set_launch(self, waves_per_group: int = 0, use_graph: int = 0, blocks: int = 0, sort_by_cost: int = 1):
+    def set_launch(self, waves_per_group: int = 0, use_graph: int = 0, blocks: int = 0, sort_by_cost: int = 1,
+                   teams_per_block: int = 0):
         """Launch shape: waves per trajectory group G (0 = auto), HIP-graph
         replay, persistent grid size (0 = fill the device), hand-out order
         (0 input order, 1 longest first by the learned cost model, 2 longest
-        first by revolutions)."""
-        L = nat.Launch(waves_per_group, use_graph, blocks, sort_by_cost)
+        first by revolutions), and the team kernel's groups per workgroup
+        (0 = the fixed-G kernel; 1, 2 or 4 = the team kernel, whose results
+        do not depend on the launch shape or the batch, with G = 0, FP64 or
+        Encke, fields of degree >= 30)."""
+        L = nat.Launch(waves_per_group, use_graph, blocks, sort_by_cost, teams_per_block)
         self._check(self._lib.bqp_set_launch(self._ctx, C.byref(L)))
         self.launch = {"waves_per_group": waves_per_group, "use_graph": use_graph, "blocks": blocks,
-                       "sort_by_cost": sort_by_cost}
+                       "sort_by_cost": sort_by_cost, "teams_per_block": teams_per_block}
 
     def cost_model(self) -> dict:
         """The learned cost model (sort_by_cost = 1): per (altitude bin,

@@ -128,6 +128,12 @@ typedef struct bqp_launch {
                                    revolutions x evaluations per revolution learned from
                                    the context's earlier calls (bqp_get_cost_model);
                                2 = descending revolutions (nothing learned) */
+  int32_t teams_per_block;  /* 0: the fixed-G kernel.  1, 2 or 4 (with waves_per_group = 0, FP64 or
+                               Encke, field degree >= 30): the team kernel, one 16-wave workgroup
+                               per CU carrying that many 64-trajectory groups, whose waves take the
+                               gravity work of whichever groups are still running; its results
+                               do not depend on grid, group width or batch composition (bit for
+                               bit), but it measures slower on the 65,536-row headline batch */
 } bqp_launch;
 
 /* Cells of the learned cost model: BQP_COST_ALT_BINS log-spaced bins of the
