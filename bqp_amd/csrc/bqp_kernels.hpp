@@ -1,10 +1,14 @@
 // Persistent batched propagation kernel (gfx950).
 //
 // One trajectory per lane.  A workgroup is G waves that share one group of
-// 64 trajectories: every wave carries the full integrator state of the 64
-// lanes, computes the gravity columns it owns (col_owner), and the G partial
-// accelerations are summed through LDS in a fixed order, so all G waves hold
-// bit-identical state and take identical (wave-uniform) control flow.
+// 64 trajectories.  Each wave computes the gravity columns it owns
+// (col_owner) and the G partial accelerations are summed through LDS in a
+// fixed order.  In the FP64, split and Encke forms wave 0 alone carries the
+// integrator state of the 64 lanes (stage sums, error norm, controller,
+// refill) and hands each evaluation's body-frame positions to the other
+// waves (grav_worker); in the surrogate forms every wave carries the state,
+// bit-identical, and takes identical (wave-uniform) control flow, because the
+// network tiles are dealt to all G waves.
 //
 // Lanes pull rows from a device work queue (one atomic per wave per refill,
 // ranks by mbcnt), so trajectories of very different lengths share one launch;
@@ -337,20 +341,56 @@ __device__ __forceinline__ void battin(double mu, const double* rb, const double
   az += k * fma(fq, rb[2], -dr[2]);
 }
 
+// Wave 0 alone runs the integrator when the group has more than one wave and
+// no network (PREC 0, 1, 4): the other waves only compute gravity columns.
+template <int PREC, int G>
+constexpr bool leader_mode() {
+  return G > 1 && PREC != 2 && PREC != 3;
+}
+
+// A non-leading wave of a leader-mode group.  Each round wave 0 publishes the
+// body-frame positions of its 64 lanes in sP and passes barrier A; this wave
+// computes its gravity columns, leaves the partial sum in lds[g] and passes
+// barrier B, after which wave 0 adds the G partials in wave order.  sCmd = 0
+// at barrier A ends the loop.  sP is rewritten and lds[g] read by wave 0
+// only between B and the next A, so two barriers per evaluation suffice.
+template <int PREC, int G>
+__device__ __forceinline__ void grav_worker(const PropParams& p, int g, int lane, double (*lds)[3][64],
+                                            const double (*sP)[64], const int* sCmd) {
+  while (true) {
+    __syncthreads();  // A
+    if (__builtin_amdgcn_readfirstlane(*sCmd) == 0) return;
+    const double xb = sP[0][lane], yb = sP[1][lane], zb = sP[2][lane];
+    double gx, gy, gz;
+    if (PREC == 1)
+      grav_split<G>(p.field, g, xb, yb, zb, gx, gy, gz);
+    else
+      grav_fp64<G>(p.field, g, xb, yb, zb, gx, gy, gz);
+    lds[g][0][lane] = gx;
+    lds[g][1][lane] = gy;
+    lds[g][2][lane] = gz;
+    __syncthreads();  // B
+  }
+}
+
 // Inertial Cowell RHS: uniform z-rotation to the body frame, gravity, rotate
 // back, exponential drag (LY/src/rhs.rs:1119-1163 compute_drag).  The
 // acceleration is computed by wave 0 only and, for lanes with `store`,
-// written into the group's LDS stage slot sA[slot]; with G > 1 the trailing
-// barrier both frees the partial-sum buffer and publishes sA[slot] to the
-// other waves, which only contribute gravity partial sums.
+// written into the group's LDS stage slot sA[slot].  In leader mode the
+// other waves sit in grav_worker and see only sP and their lds[g].  Without
+// it (surrogate forms) every wave carries the integrator state and calls
+// this function; the trailing barrier then both frees the partial-sum buffer
+// and publishes sA[slot] to the other waves, which only contribute gravity
+// partial sums.
 template <int PREC, int G, int S>
 __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
                                            double (*lds)[3][64], double (*sA)[3][64], int slot,
                                            bool store, double bc, double t, const double* y,
                                            double* acc, uint4* sX, float* sO, int* sFlag, SurLane& sl,
-                                           EnckeLane& el) {
+                                           EnckeLane& el, double (*sP)[64]) {
   constexpr bool SUR = (PREC == 2 || PREC == 3);
   constexpr bool ENCKE = (PREC == 4);
+  constexpr bool LEAD = leader_mode<PREC, G>();
   // total (inertial) state: y itself, or baseline + deviation (Encke)
   double yt[6], bst[6];
   if constexpr (ENCKE) {
@@ -367,6 +407,12 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
   const double xb = fma(c, yt[0], s * yt[1]);
   const double yb = fma(c, yt[1], -(s * yt[0]));
   const double zb = yt[2];
+  if constexpr (LEAD) {
+    sP[0][lane] = xb;
+    sP[1][lane] = yb;
+    sP[2][lane] = zb;
+    __syncthreads();  // A: the workers start their columns
+  }
   double gx, gy, gz;
   // surrogate decisions (identical in every wave of the group)
   bool band = true, use_net = false, check = false;
@@ -385,7 +431,9 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
     gx = gy = gz = 0.0;
     if (wave_ballot(need_exact) != 0ull) grav_fp64<G>(p.field, g, xb, yb, zb, gx, gy, gz);
   }
-  if (G > 1) {
+  if constexpr (LEAD) {
+    __syncthreads();  // B: the workers' partials are in lds[1..G-1]
+  } else if (G > 1) {
     lds[g][0][lane] = gx;
     lds[g][1][lane] = gy;
     lds[g][2][lane] = gz;
@@ -393,6 +441,8 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
     gx = lds[0][0][lane];
     gy = lds[0][1][lane];
     gz = lds[0][2][lane];
+  }
+  if (G > 1) {
 #pragma unroll
     for (int q = 1; q < G; ++q) {
       gx += lds[q][0][lane];
@@ -402,7 +452,7 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
   }
   // only wave 0 publishes the stage acceleration; the other waves' copies
   // would be identical, so they skip the inertial rotation and the drag
-  if (G > 1 && g != 0) {
+  if (!LEAD && G > 1 && g != 0) {
     __syncthreads();
     if (SUR) sl.exact = sFlag[lane] != 0;  // wave 0's verdict of this evaluation's check
     return;
@@ -446,7 +496,7 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
     sA[slot][1][lane] = ay;
     sA[slot][2][lane] = az;
   }
-  if (G > 1) __syncthreads();
+  if (!LEAD && G > 1) __syncthreads();
 }
 
 template <int METHOD, int PREC, int G>
@@ -468,6 +518,18 @@ propagate_kernel(const PropParams p) {
   __shared__ uint4 sX[SUR ? 64 * 4 : 1];
   __shared__ float sO[SUR ? 64 * 3 : 1];
   __shared__ int sFlag[SUR && G > 1 ? 64 : 1];
+  constexpr bool LEAD = leader_mode<PREC, G>();
+  // leader mode: body-frame positions wave 0 hands to the other waves, and
+  // its go (1) / stop (0) word
+  __shared__ double sP[LEAD ? 3 : 1][64];
+  __shared__ int sCmd;
+  if constexpr (LEAD) {
+    if (g != 0) {
+      grav_worker<PREC, G>(p, g, lane, lds, sP, &sCmd);
+      return;
+    }
+    if (lane == 0) sCmd = 1;
+  }
   SurLane sl;
   EnckeLane el{};
 
@@ -571,7 +633,7 @@ propagate_kernel(const PropParams p) {
         // every lane is free: the block's first group is dealt, not queued
         base = 64u * first_group(blockIdx.x, gridDim.x, (unsigned)p.deal_cus);
         first_fill = false;
-      } else if (G == 1) {
+      } else if (G == 1 || LEAD) {
         unsigned int b = 0;
         if (lane == __ffsll((long long)need) - 1) b = atomicAdd(p.queue, (unsigned)cnt);
         base = __shfl(b, __ffsll((long long)need) - 1);
@@ -694,7 +756,7 @@ propagate_kernel(const PropParams p) {
       }
       double as[3];
       if (SUR) sl.check_now = p.sur_check_every > 0 && (evals % p.sur_check_every) == 0;
-      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, st, bc, t + T.c[s] * hs, ys, as, sX, sO, sFlag, sl, el);
+      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, st, bc, t + T.c[s] * hs, ys, as, sX, sO, sFlag, sl, el, sP);
       if (st) ++evals;
     }
     if (needk) have_k1 = true;
@@ -762,7 +824,7 @@ propagate_kernel(const PropParams p) {
     // FSAL: on accept, wave 0 overwrites sA[0] with the last stage for the
     // next step; every wave must have read sA[0] for the sums above first
     // (a wave that lags behind wave 0 would otherwise read the new value)
-    if (MT::FSAL && G > 1) __syncthreads();
+    if (MT::FSAL && G > 1 && !LEAD) __syncthreads();
     bool finite = true;
 #pragma unroll
     for (int i = 0; i < 6; ++i) finite = finite && isfinite(yn[i]) && isfinite(err[i]);
@@ -913,7 +975,11 @@ propagate_kernel(const PropParams p) {
       }
     }
     // publish the FSAL copy of sA[0] before the next step reads it
-    if (MT::FSAL && G > 1) __syncthreads();
+    if (MT::FSAL && G > 1 && !LEAD) __syncthreads();
+  }
+  if constexpr (LEAD) {
+    if (lane == 0) sCmd = 0;
+    __syncthreads();  // A: releases the workers
   }
   if (SUR && g == 0) {
     unsigned long long v[5] = {sl.net, sl.exact_n, sl.checks, sl.fails, sl.band};
