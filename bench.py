@@ -311,6 +311,136 @@ def run_surrogate_study(args) -> dict:
     }
 
 
+def _round_bf16(x: np.ndarray) -> np.ndarray:
+    """float32 -> nearest bfloat16 (ties to even) -> float32."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def run_floors(args) -> dict:
+    """Error floors of the two reduced-precision forms on the headline field
+    (Kaula degree 70, LEO band 6,578-8,378 km, RK8(7)13M rtol 1e-12).
+
+    Learned surrogate: the network trained from scratch for each step budget
+    in --floor-steps (10M samples, batch 8,192, cosine learning rate); on 1M
+    held-out samples, in m/s^2: the FP32 network against the exact residual
+    (the fitting floor), the FP32 network on BF16-rounded features against
+    the same network on FP32 features (the input-quantisation floor, which
+    the BF16 and FP8 forms share: both read BF16 features), and the BF16 /
+    FP8 MFMA networks against the FP32 network (all quantisation together).
+
+    Split form: on one accepted-size step of each of 4,096 rows (the mean
+    step of that row's 1-day FP64 run), the change of the embedded error
+    estimate when every stage acceleration comes from the split-precision
+    gravity instead of the FP64 one, in units of the WRMS error norm
+    (scale atol + rtol |y| per component, as the kernel's controller), next
+    to the FP64 estimate itself; and the RHS evaluations of the split form
+    over the same day against FP64's."""
+    import bqp_amd
+    from bqp_amd import inputs
+    from bqp_amd.device import DeviceArray, synchronize
+    from bqp_amd.propagator import tableau
+
+    field = bqp_amd.kaula(70)
+    prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12)
+    out = {"metric": "surrogate and split-form error floors", "field": "Kaula degree/order 70 (seed 0)"}
+
+    # ------------------------------------------------------------ surrogate
+    budgets = [int(s) for s in args.floor_steps.split(",") if s]
+    sur = bqp_amd.Surrogate(prop, nlow=4, r_lo=6578.0e3, r_hi=8378.0e3, seed=0)
+    feat, targ, _ = sur.dataset(args.sur_rows, seed=1)
+    vfeat, vtarg, _ = sur.dataset(1_000_000, seed=977)
+    T = vtarg.numpy()
+    vfeat_b = DeviceArray.from_numpy(_round_bf16(vfeat.numpy()))
+    calib = feat.rows(0, 65536)
+    curve = []
+    for steps in budgets:
+        sur = bqp_amd.Surrogate(prop, nlow=4, r_lo=6578.0e3, r_hi=8378.0e3, seed=0)
+        scale = sur.out_scale
+        t = time.perf_counter()
+        losses = sur.fit(feat, targ, steps=steps, batch=8192, lr_max=2e-3, lr_min=1e-5, chunk=500, seed=0)
+        synchronize()
+        train_s = time.perf_counter() - t
+        o32 = sur.infer(vfeat, "fp32")[0].numpy()
+        o32b = sur.infer(vfeat_b, "fp32")[0].numpy()
+        sur.pack("bf16")
+        obf = sur.infer(vfeat, "bf16")[0].numpy()
+        sur.pack("fp8", calib=calib)
+        o8 = sur.infer(vfeat, "fp8")[0].numpy()
+
+        def rms(d):
+            e = np.linalg.norm(d * scale, axis=1)
+            return {"rms": float(np.sqrt(np.mean(e ** 2))), "median": float(np.median(e)), "max": float(e.max()),
+                    "frac_le_1e-9": float(np.mean(e <= 1e-9))}
+
+        rec = {"steps": steps, "train_s": train_s, "loss_last": float(np.mean(losses[-50:])),
+               "fit_fp32_vs_truth": rms(o32 - T), "input_bf16_vs_fp32": rms(o32b - o32),
+               "bf16_vs_fp32": rms(obf - o32), "fp8_vs_fp32": rms(o8 - o32), "bf16_vs_truth": rms(obf - T)}
+        curve.append(rec)
+        print(f"[bench] floors: {steps} steps ({train_s:.1f} s): fit {rec['fit_fp32_vs_truth']['rms']:.3e}, "
+              f"input bf16 {rec['input_bf16_vs_fp32']['rms']:.3e}, bf16 {rec['bf16_vs_fp32']['rms']:.3e}, "
+              f"fp8 {rec['fp8_vs_fp32']['rms']:.3e} m/s^2 RMS", file=sys.stderr, flush=True)
+    res = np.linalg.norm(T * sur.out_scale, axis=1)
+    out["surrogate"] = {"residual_rms_m_s2": float(np.sqrt(np.mean(res ** 2))), "check_bound_m_s2": 1e-9,
+                        "relative_accuracy_needed": 1e-9 / float(np.sqrt(np.mean(res ** 2))),
+                        "budgets": curve}
+
+    # ------------------------------------------------------------ split form
+    n = 4096
+    w = inputs.workload("w1", n=n)
+    tb = tableau("rk87")
+    S, A, c, e = tb["stages"], tb["a"], tb["c"], tb["e"]
+    evals = {}
+    hmean = None
+    for prec in ("fp64", "split"):
+        with bqp_amd.Propagator(field, method="rk87", rtol=1e-12, precision=prec) as p:
+            _, st, info = p.propagate_batch_final(w["y0"], 0.0, np.full(n, 86400.0))
+            evals[prec] = int(np.sum(info["evals_per_row"]))
+            if prec == "fp64":
+                hmean = 86400.0 / np.maximum(info["steps_per_row"], 1)
+    cfg = prop.config
+    with bqp_amd.Propagator(field, method="rk87", rtol=1e-12, precision="split") as ps:
+        y = w["y0"].copy()
+        h = hmean[:, None]
+        K64 = np.zeros((S, n, 6))
+        dA = np.zeros((S, n, 3))
+        for s in range(S):
+            ys = y + h * np.einsum("j,jnk->nk", A[s, :s], K64[:s]) if s else y.copy()
+            a64 = prop.gravity_accel(ys[:, :3])
+            a32 = ps.gravity_accel(ys[:, :3])
+            K64[s, :, :3] = ys[:, 3:]
+            K64[s, :, 3:] = a64
+            dA[s] = a32 - a64
+    err64 = h * np.einsum("j,jnk->nk", e, K64)
+    # the stage velocities inherit the acceleration noise: V_s = v + h sum_j a_sj A_j
+    dK = np.zeros((S, n, 6))
+    for s in range(S):
+        dv = h * np.einsum("j,jnk->nk", A[s, :s], dA[:s]) if s else np.zeros((n, 3))
+        dK[s, :, :3] = dv
+        dK[s, :, 3:] = dA[s]
+    derr = h * np.einsum("j,jnk->nk", e, dK)
+    atol = np.array([cfg.atol_pos] * 3 + [cfg.atol_vel] * 3)
+    sc = atol + cfg.rtol * np.abs(y)  # the step's end state differs by O(h |y'|) -- immaterial here
+    wrms = lambda d: np.sqrt(np.mean((d / sc) ** 2, axis=1))
+    en64, enoise = wrms(err64), wrms(derr)
+    dA_norm = np.linalg.norm(dA, axis=2)
+    out["split"] = {"rows": n, "rtol": cfg.rtol, "atol_pos_m": cfg.atol_pos, "atol_vel_m_s": cfg.atol_vel,
+                    "mean_step_s": {"median": float(np.median(hmean)), "min": float(hmean.min()),
+                                    "max": float(hmean.max())},
+                    "gravity_noise_m_s2": {"rms": float(np.sqrt(np.mean(dA_norm ** 2))), "max": float(dA_norm.max())},
+                    "en_fp64": {"median": float(np.median(en64)), "p90": float(np.percentile(en64, 90))},
+                    "en_noise": {"median": float(np.median(enoise)), "p90": float(np.percentile(enoise, 90)),
+                                 "max": float(enoise.max())},
+                    "vel_noise_over_scale_median": float(np.median(np.abs(derr[:, 3:]) / sc[:, 3:])),
+                    "pos_noise_over_scale_median": float(np.median(np.abs(derr[:, :3]) / sc[:, :3])),
+                    "evals_1day": evals, "evals_ratio": evals["split"] / evals["fp64"]}
+    print(f"[bench] floors: split noise en median {np.median(enoise):.3g} (fp64 en median {np.median(en64):.3g}), "
+          f"evals ratio {evals['split'] / evals['fp64']:.3f}", file=sys.stderr, flush=True)
+    prop.close()
+    return out
+
+
 def run_population(args) -> dict:
     """Population evaluation: 32 NSGA-II generations x 4,096 candidates
     (131,072 arcs) over seeded LEO departure states with dv U(-50, 50) m/s
@@ -547,10 +677,14 @@ def main():
                          "calibrating HBM byte counters)")
     ap.add_argument("--pop", type=int, default=4096, help="w4: population size")
     ap.add_argument("--generations", type=int, default=32, help="w4: generations (incl. the initial one)")
-    ap.add_argument("--workload", default="w1", choices=["w1", "w2", "w3", "w4"],
+    ap.add_argument("--floor-steps", default="1000,3000,10000,30000",
+                    help="floors: training step budgets of the surrogate's error-floor curve")
+    ap.add_argument("--workload", default="w1", choices=["w1", "w2", "w3", "w4", "floors"],
                     help="w1: 65,536 LEO x 7 d deg-70 (the headline); w2: ragged 262,144 mixed orbits deg-20; "
                          "w3: the surrogate study (training, MFMA inference rooflines, fallback, ablation); "
-                         "w4: NSGA-II population evaluation (32 x 4,096 arcs, deg-50 + drag)")
+                         "w4: NSGA-II population evaluation (32 x 4,096 arcs, deg-50 + drag); "
+                         "floors: error floors of the surrogate (fitting, input and weight quantisation) and of "
+                         "the split form's error estimate")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the propagation path runs on a single MI355X "
@@ -569,6 +703,9 @@ def main():
         return
     if args.workload == "w4":
         print(json.dumps(run_population(args)))
+        return
+    if args.workload == "floors":
+        print(json.dumps(run_floors(args)))
         return
 
     import bqp_amd
