@@ -25,8 +25,9 @@ sample of the same workload, run before any GPU work.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N]
        [--days D] [--forms fp64,split,surrogate,encke] [--no-cpu]
-       [--budget-s S] [--workload w1|w2|w3|w4]
-       (w2: ragged batch; w3: surrogate study; w4: NSGA-II population)
+       [--budget-s S] [--workload w1|w2|w3|w4|floors]
+       (w2: ragged batch; w3: surrogate study; w4: NSGA-II population;
+        floors: error floors of the surrogate and of the split form)
 """
 from __future__ import annotations
 
@@ -329,6 +330,9 @@ def run_floors(args) -> dict:
     the same network on FP32 features (the input-quantisation floor, which
     the BF16 and FP8 forms share: both read BF16 features), and the BF16 /
     FP8 MFMA networks against the FP32 network (all quantisation together).
+    Then the network trained on degree-120 truth (the second budget): the
+    size of degrees 71-120 at the same positions, and its error against
+    degree 120 and against the degree-70 field the propagation uses.
 
     Split form: on one accepted-size step of each of 4,096 rows (the mean
     step of that row's 1-day FP64 run), the change of the embedded error
@@ -381,10 +385,39 @@ def run_floors(args) -> dict:
         print(f"[bench] floors: {steps} steps ({train_s:.1f} s): fit {rec['fit_fp32_vs_truth']['rms']:.3e}, "
               f"input bf16 {rec['input_bf16_vs_fp32']['rms']:.3e}, bf16 {rec['bf16_vs_fp32']['rms']:.3e}, "
               f"fp8 {rec['fp8_vs_fp32']['rms']:.3e} m/s^2 RMS", file=sys.stderr, flush=True)
-    res = np.linalg.norm(T * sur.out_scale, axis=1)
+    scale70 = sur.out_scale
+    res = np.linalg.norm(T * scale70, axis=1)
     out["surrogate"] = {"residual_rms_m_s2": float(np.sqrt(np.mean(res ** 2))), "check_bound_m_s2": 1e-9,
                         "relative_accuracy_needed": 1e-9 / float(np.sqrt(np.mean(res ** 2))),
                         "budgets": curve}
+
+    # Training truth of degree 120 (the Kaula field of degree 70 is a prefix of
+    # the degree-120 one): the network learns degrees 5..120 while the
+    # propagation field, its exact fallback and its check stop at 70.
+    steps120 = budgets[min(1, len(budgets) - 1)]
+    p120 = bqp_amd.Propagator(bqp_amd.kaula(120), method="rk87", rtol=1e-12)
+    s120 = bqp_amd.Surrogate(p120, nlow=4, r_lo=6578.0e3, r_hi=8378.0e3, seed=0)
+    f120, t120, _ = s120.dataset(args.sur_rows, seed=1)
+    v120f, v120t, _ = s120.dataset(1_000_000, seed=977)
+    same_positions = bool(np.array_equal(v120f.numpy(), vfeat.numpy()))
+    s120.fit(f120, t120, steps=steps120, batch=8192, lr_max=2e-3, lr_min=1e-5, chunk=500, seed=0)
+    o120 = s120.infer(v120f, "fp32")[0].numpy() * s120.out_scale
+    T120 = v120t.numpy() * s120.out_scale
+    T70 = T * scale70
+
+    def stat(d):
+        e = np.linalg.norm(d, axis=1)
+        return {"rms": float(np.sqrt(np.mean(e ** 2))), "median": float(np.median(e)), "max": float(e.max())}
+
+    out["surrogate"]["truth_degree_120"] = {
+        "steps": steps120, "same_positions": same_positions,
+        "degrees_71_120_m_s2": stat(T120 - T70),
+        "fit_fp32_vs_degree_120": stat(o120 - T120),
+        "fp32_vs_degree_70_field": stat(o120 - T70)}
+    print(f"[bench] floors: degree-120 truth ({steps120} steps): degrees 71-120 part "
+          f"{stat(T120 - T70)['rms']:.3e}, fit vs 120 {stat(o120 - T120)['rms']:.3e}, vs the degree-70 field "
+          f"{stat(o120 - T70)['rms']:.3e} m/s^2 RMS", file=sys.stderr, flush=True)
+    p120.close()
 
     # ------------------------------------------------------------ split form
     n = 4096
