@@ -424,11 +424,31 @@ int grid_for(bqp_ctx* c, prop_fn kern, int G, int64_t n, int* blocks) {
 // The team kernel (bqp_team.hpp) takes FP64 Cowell and Encke calls that ask
 // for it (teams_per_block 1, 2 or 4 with an automatic group width) on fields
 // of degree >= PIPE_MIN_N: one 16-wave workgroup per CU carrying that many
-// groups.  Other calls take the fixed-G kernel.
+// groups.  The pair kernel takes FP64 Cowell and Encke calls with
+// waves_per_group 8 and teams_per_block 2: two groups per 8-wave workgroup.
+// Other calls take the fixed-G kernel.
 int shape_for(bqp_ctx* c, int kprec, int64_t n, LaunchShape* out) {
   const bqp_launch& L = c->launch;
   const int64_t groups = (n + 63) / 64;
   *out = LaunchShape{};
+  if (L.waves_per_group == PAIR_WAVES && L.teams_per_block == PAIR_GROUPS && (kprec == 0 || kprec == 4)) {
+    out->kern = (kprec == 4) ? pick_pair_encke(c->cfg.method) : pick_pair_fp64(c->cfg.method);
+    if (!out->kern) return fail(c, BQP_E_INVALID, "unknown method");
+    const int64_t need = (groups + PAIR_GROUPS - 1) / PAIR_GROUPS;
+    int64_t b = L.blocks;
+    if (b <= 0) {
+      int per_cu = 0;
+      HIP_TRY(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)out->kern, 64 * PAIR_WAVES, 0));
+      b = (int64_t)std::max(per_cu, 1) * c->cus;
+    }
+    b = std::max<int64_t>(1, std::min<int64_t>(b, need));
+    out->threads = 64 * PAIR_WAVES;
+    out->blocks = (int)b;
+    out->G = PAIR_WAVES;
+    out->teams = PAIR_GROUPS;
+    out->queue0 = (unsigned)std::min<int64_t>(n, 64LL * PAIR_GROUPS * b);
+    return BQP_OK;
+  }
   const bool team = L.waves_per_group == 0 && L.teams_per_block > 0 && (kprec == 0 || kprec == 4) &&
                     c->nmax >= PIPE_MIN_N;
   if (team) {
@@ -863,6 +883,11 @@ int32_t bqp_set_launch(bqp_ctx* c, const bqp_launch* l) {
   if (l->sort_by_cost < 0 || l->sort_by_cost > 2) return fail(c, BQP_E_INVALID, "sort_by_cost must be 0, 1 or 2");
   if (l->teams_per_block != 0 && l->teams_per_block != 1 && l->teams_per_block != 2 && l->teams_per_block != 4)
     return fail(c, BQP_E_INVALID, "teams_per_block must be 0 (the fixed-G kernel), 1, 2 or 4");
+  if (l->teams_per_block != 0 && l->waves_per_group != 0 &&
+      !(l->waves_per_group == PAIR_WAVES && l->teams_per_block == PAIR_GROUPS))
+    return fail(c, BQP_E_INVALID,
+                "with teams_per_block > 0, waves_per_group must be 0 (the team kernel) or 8 with teams_per_block 2 "
+                "(the pair kernel)");
   (void)hipStreamSynchronize(c->stream);
   bqp_drop_graph(c);
   c->launch = *l;

@@ -1,44 +1,43 @@
-// Team kernel: the persistent batched propagation with the gravity work of
-// one CU shared out among its trajectory groups as they finish (gfx950).
+// Shared-workgroup propagation kernels (gfx950): several 64-trajectory groups
+// in one workgroup, whose waves move to the groups still running as the
+// others finish.
 //
-// One workgroup of TEAM_WAVES = 16 waves per CU (four per SIMD) carries up to
-// TEAM_MAX = 4 groups of 64 trajectories ("teams"), one trajectory per lane.
-// Wave t < NT is the leader of team t and alone holds that team's integrator
-// state; every wave, leaders included, computes gravity.  Each loop
-// iteration is one right-hand-side evaluation for every live team:
-//   A  each leader starts a step if its team has none in flight (refilling
-//      rows), then writes the team's next stage state, frame angle and
-//      Cd*A/m to LDS, and its liveness;
+// Both kernels run the per-row integrator of propagate_kernel
+// (bqp_kernels.hpp) in a leader wave per group, as a per-evaluation state
+// machine (GroupLeader below): rk_step and the controller restated from
+// LY/src/odesolve/solver.rs:445-529, 1099-1925, 1965-2173 and
+// LY/src/odesolve/lightyear_compat.rs:100-600, the Encke segment driver of
+// LY/src/integrator.rs:341-402, 3418-3563.  Each loop iteration is one
+// right-hand-side evaluation of every live group:
+//   A  each leader starts a step if its group has none in flight (refilling
+//      rows) and publishes the group's next stage in LDS;
 //   -- barrier --
-//   B  every wave takes work items from one LDS counter until none are
-//      left: an item is one gravity unit of a live team (see grav_unit;
-//      two or fewer live teams) or an aligned pair of units (three or four),
-//      so 16 waves always find at least 16 items.  The wave turns the team's
-//      stage position into the body frame (sincos of the team's angle),
-//      sums the item's units, turns each back to the inertial frame and
-//      writes the item's sum to its LDS slot; the wave that takes a team's
-//      first item also computes the team's drag;
+//   B  every wave, leaders included, computes gravity work items of the live
+//      groups into LDS slots;
 //   -- barrier --
-//   C  each leader combines its team's slots along the fixed pairwise tree
-//      over the 16 units, adds the drag (and the Encke terms), stores the
-//      stage and, after the last stage, completes the step (solution, error
-//      norm, controller, row results).
-// When a team runs out of rows its items leave the pool, so the CU's waves
-// all work on the groups still running until the last row of the CU ends.
-// Every unit is computed the same way whichever wave takes it, and the tree
-// is fixed, so a team's accelerations, and the results, do not depend on the
-// number of live teams, the grid, the hand-out order or the batch a row is
-// part of.  The price is lockstep: each evaluation waits for the last item
-// and for the leaders' serial work, which on the 65,536-row headline batch
-// costs more than the tail it removes, so the fixed-G kernel stays the
-// default and this one runs on request (bqp_launch.teams_per_block).
+//   C  each leader combines its group's slots in a fixed order, adds drag
+//      (and the Encke terms), stores the stage and, after the last stage,
+//      completes the step (solution, error norm, controller, row results).
+// A group whose rows are done leaves the pool, so its waves work on the
+// groups still running until the workgroup's last row ends.  The price is
+// lockstep: each evaluation waits for the last item and for the slowest
+// leader's serial work.
 //
-// The per-row integrator is the one of propagate_kernel (bqp_kernels.hpp):
-// rk_step and the controller restated from LY/src/odesolve/solver.rs:
-// 445-529, 1099-1925, 1965-2173 and LY/src/odesolve/lightyear_compat.rs:
-// 100-600, the Encke segment driver of LY/src/integrator.rs:341-402,
-// 3418-3563; it is run here by the leader alone as a per-evaluation state
-// machine (the stage index s_cur).
+// Team kernel (propagate_team_kernel): one 16-wave workgroup per CU, up to
+// four groups ("teams").  A work item is one of 16 fixed gravity units of a
+// team (grav_unit), or an aligned pair of units when more than two teams are
+// live, taken from an LDS counter; each unit is turned back to the inertial
+// frame on its own and the units are combined along a fixed pairwise tree,
+// so the results do not depend on the number of live teams, the grid, the
+// hand-out order or the batch (bit for bit).
+//
+// Pair kernel (propagate_pair_kernel): 8-wave workgroups, two per CU, each
+// carrying two groups.  A group's gravity is the eight column shares of the
+// fixed-G kernel at G = 8 (col_owner<8>), summed in share order in the body
+// frame, so its results equal waves_per_group = 8 bit for bit.  While both
+// groups run each wave computes two shares of one group (the work of a wave
+// of the G = 4 kernel); when one is done its four waves join the other,
+// which then runs at the G = 8 latency.
 #pragma once
 
 #include "bqp_kernels.hpp"
@@ -52,48 +51,27 @@ constexpr int TEAM_MAX = 4;
 constexpr int TEAM_PAIRS_ABOVE = 2;
 constexpr int TEAM_SLOTS = 2 * GRAV_UNITS;
 
-// First group (in hand-out positions) of team t of block b: the teams of the
-// blocks are dealt in rounds, round t over blocks 0..C-1 in order, odd rounds
-// in reverse, so each block's teams take ranks b, 2C-1-b, 2C+b, 4C-1-b of the
-// longest-first order and the CUs carry nearly equal work.
-__device__ __forceinline__ unsigned team_first_group(unsigned b, unsigned t, unsigned C) {
-  return t * C + ((t & 1u) ? (C - 1u - b) : b);
-}
+// Pair kernel: waves per workgroup = column shares per group, and groups.
+constexpr int PAIR_WAVES = 8;
+constexpr int PAIR_GROUPS = 2;
 
+// The integrator state of one group's 64 rows, held by the group's leader
+// wave, and its per-evaluation steps.  sA is the group's LDS stage storage.
 template <int METHOD, int PREC>
-__global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const PropParams p) {
+struct GroupLeader {
   using MT = MethodTraits<METHOD>;
-  constexpr int S = MT::S;
-  constexpr bool ENCKE = (PREC == 4);
-  static_assert(PREC == 0 || PREC == 4, "team kernel: plain FP64 Cowell or Encke");
-  const bqp_tableau& TB = c_tab[METHOD];
-  const SecondOrderTableau& SO = c_so[METHOD];
-  const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int NT = p.teams;  // 1, 2 or 4 teams per block
-  const bool leader = w < NT;
-  const int team = w;
+  static constexpr int S = MT::S;
+  static constexpr bool ENCKE = (PREC == 4);
 
-  __shared__ double sA[TEAM_MAX][S][3][64];   // stage accelerations of each team
-  __shared__ double sYt[TEAM_MAX][6][64];     // inertial stage states (total state for Encke)
-  __shared__ double sTh[TEAM_MAX][64];        // body-frame angle at the stage time
-  __shared__ double sBc[TEAM_MAX][64];        // Cd*A/m of the lanes that store the stage (else 0)
-  __shared__ double sDrag[TEAM_MAX][3][64];   // drag accelerations of the stage
-  __shared__ double sSlot[TEAM_SLOTS][3][64]; // gravity subtree sums of the iteration's work items
-  __shared__ int sAlive[TEAM_MAX];
-  __shared__ int sNext;                       // next work item of the iteration
-
-  const bool compat = (p.control == CONTROL_COMPAT);
-  const bool sampled = (p.t_eval != nullptr);
-  const double q = compat ? 5.0 : (double)(TB.order_err + 1);
-  const double inv_q = 1.0 / q;
-
-  // ---- leader state (the team's 64 rows)
-  bool alive = leader, in_step = false, exhausted = false, first_fill = true;
-  bool active = false;
+  const PropParams& p;
+  double (*sA)[3][64];
+  int lane;
+  bool compat, sampled;
+  double q, inv_q;
+  bool alive, in_step = false, exhausted = false, first_fill = true, active = false;
   int row = 0, steps = 0, evals = 0, rejects = 0, rej_run = 0, sidx = 0;
   double t = 0.0, tseg = 0.0, tend = 0.0, h = 0.0, tcomp = 0.0, eprev = 0.0, bc = 0.0;
-  double hmin = 0.0, hmax = 0.0, eps = p.eps;
+  double hmin = 0.0, hmax = 0.0, eps;
   bool have_prev = false, just_rej = false, have_k1 = false;
   double y[6] = {0, 0, 0, 0, 0, 0};
   EnckeLane el{};
@@ -101,10 +79,18 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
   int s_cur = 0;
   bool stepping = false, needk = false, lands = false, st = false;
   double hs = 0.0;
-  double bst[3] = {0, 0, 0}, ysr[3] = {0, 0, 0};
-  const bool drag_on = p.model.drag != DRAG_NONE;
+  double bst[3] = {0, 0, 0}, ysr[3] = {0, 0, 0};  // Encke: stage baseline position and deviation
 
-  auto finish = [&](int code) {
+  __device__ __forceinline__ GroupLeader(const PropParams& p_, double (*sA_)[3][64], int lane_, bool alive_)
+      : p(p_), sA(sA_), lane(lane_), alive(alive_), eps(p_.eps) {
+    compat = (p.control == CONTROL_COMPAT);
+    sampled = (p.t_eval != nullptr);
+    q = compat ? 5.0 : (double)(c_tab[METHOD].order_err + 1);
+    inv_q = 1.0 / q;
+  }
+
+  // Writes the lane's row result and frees the lane.
+  __device__ __forceinline__ void finish(int code) {
     if (code != ROW_OK) {
       if (sampled) {
         for (int m = sidx; m < p.m_eval; ++m)
@@ -120,9 +106,11 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
     if (p.evals) p.evals[row] = evals;
     if (p.rejects) p.rejects[row] = rejects;
     active = false;
-  };
+  }
 
-  auto start_controller = [&](double span) {
+  // Step-size bounds and initial step of an integration over `span`
+  // (solver.rs:385-391 default h0; lightyear_compat.rs:203-213 for compat).
+  __device__ __forceinline__ void start_controller(double span) {
     if (compat) {
       eps = fmax(p.eps, 1e-12);
       hmin = 1e-10;
@@ -135,9 +123,11 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
       const double h0 = (p.h0 > 0.0) ? p.h0 : ((span <= 60.0) ? span / 2.0 : span / 100.0);
       h = clampd(h0, hmin, hmax);
     }
-  };
+  }
 
-  auto start_segment = [&](const double* tot, double carry) {
+  // Encke: a segment from total state `tot` at the current time, opened with
+  // the carried step size `carry` when there is one (integrator.rs:341-402).
+  __device__ __forceinline__ void start_segment(const double* tot, double carry) {
 #pragma unroll
     for (int i = 0; i < 6; ++i) {
       el.base[i] = tot[i];
@@ -152,9 +142,10 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
     eprev = 0.0;
     rej_run = 0;
     tcomp = 0.0;
-  };
+  }
 
-  auto total_state = [&](double* tot) {
+  // Total (inertial) state of the lane at the current time.
+  __device__ __forceinline__ void total_state(double* tot) {
     if constexpr (ENCKE) {
       double b[6];
       two_body(p.mu, el.base, t - el.t0, b, el.x);
@@ -164,18 +155,19 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
 #pragma unroll
       for (int i = 0; i < 6; ++i) tot[i] = y[i];
     }
-  };
+  }
 
-  // Takes rows for free lanes (the dealt first group, then the queue) and
-  // sets each new row up; returns with the lanes' new rows started.
-  auto refill = [&]() {
+  // Takes rows for free lanes (the dealt first group `first_group`, then the
+  // queue; the dealt round covers `dealt_rows` hand-out positions) and sets
+  // each new row up.
+  __device__ __forceinline__ void refill(unsigned first_group, unsigned long long dealt_rows) {
     const unsigned long long need = wave_ballot(!active);
     if (need == 0ull || exhausted) return;
     const int cnt = __popcll(need);
     unsigned int base;
     const bool dealt = first_fill;
     if (first_fill) {
-      base = 64u * team_first_group(blockIdx.x, (unsigned)team, gridDim.x);
+      base = 64u * first_group;
       first_fill = false;
     } else {
       unsigned int b = 0;
@@ -184,7 +176,7 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
       base = __shfl(b, first);
     }
     if (dealt)
-      exhausted = 64ull * (unsigned long long)NT * gridDim.x >= (unsigned long long)p.n;
+      exhausted = dealt_rows >= (unsigned long long)p.n;
     else if (base + (unsigned)cnt >= (unsigned)p.n)
       exhausted = true;
     if (active) return;
@@ -235,11 +227,116 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
         start_controller(span);
       }
     }
-  };
+  }
 
-  // Completes a step attempt once all its stages are stored (rk_step's sums,
-  // the error norm and the controller), writing rows that end.
-  auto complete_step = [&]() {
+  // Phase A, first half: a group with no step in flight refills its lanes
+  // and starts a step attempt; a group with no rows left dies.
+  __device__ __forceinline__ void begin(unsigned first_group, unsigned long long dealt_rows) {
+    if (!alive || in_step) return;
+    while (true) {
+      refill(first_group, dealt_rows);
+      if (wave_ballot(active) == 0ull) {
+        if (exhausted) {
+          alive = false;
+          return;
+        }
+        continue;
+      }
+      stepping = active;
+      const double rem = tseg - t;
+      hs = (h < rem) ? h : rem;
+      lands = (hs == rem);
+      if (stepping) {
+        const int budget = compat ? steps + rejects : steps;
+        if (budget >= p.max_steps || steps + rejects >= 2 * p.max_steps) {
+          finish(ROW_MAX_STEPS);
+          stepping = false;
+        } else if (!compat && hs < hmin && !lands) {
+          finish(ROW_STEP_UNDERFLOW);
+          stepping = false;
+        }
+      }
+      if (wave_ballot(stepping) == 0ull) continue;
+      // stage 1 (k1) is kept across rejects and reused after FSAL accepts:
+      // the attempt starts at stage 0 only if some lane lacks it
+      needk = stepping && !have_k1;
+      s_cur = wave_ballot(needk) != 0ull ? 0 : 1;
+      in_step = true;
+      return;
+    }
+  }
+
+  // Phase A, second half: the total (inertial) state of stage s_cur (second-
+  // order form, as propagate_kernel) into yt; returns the stage time.
+  __device__ __forceinline__ double stage_state(double* yt) {
+    const bqp_tableau& TB = c_tab[METHOD];
+    const SecondOrderTableau& SO = c_so[METHOD];
+    const int s = s_cur;
+    st = (s == 0) ? needk : stepping;
+    double ys[6];
+    const double hh = hs * hs;
+    const double hc = hs * SO.csum[s];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      ys[i] = fma(y[3 + i], hc, y[i]);
+      ys[3 + i] = y[3 + i];
+    }
+#pragma unroll
+    for (int j = 0; j < S - 1; ++j) {
+      if (j < s) {
+        const double a = TB.a[s][j];
+        if (a != 0.0) {
+          const double sv = hs * a;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) ys[3 + i] = fma(sA[j][i][lane], sv, ys[3 + i]);
+        }
+        const double a2 = SO.a2[s][j];
+        if (a2 != 0.0) {
+          const double sr = hh * a2;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) ys[i] = fma(sA[j][i][lane], sr, ys[i]);
+        }
+      }
+    }
+    const double tst = t + TB.c[s] * hs;
+    if constexpr (ENCKE) {
+      double b6[6];
+      two_body(p.mu, el.base, tst - el.t0, b6, el.x);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) yt[i] = b6[i] + ys[i];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        bst[i] = b6[i];
+        ysr[i] = ys[i];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 6; ++i) yt[i] = ys[i];
+    }
+    return tst;
+  }
+
+  // Phase C: the stage's acceleration (central term included) is stored;
+  // after the last stage the attempt is completed.
+  __device__ __forceinline__ void store_stage(double ax, double ay, double az) {
+    if (st) {
+      sA[s_cur][0][lane] = ax;
+      sA[s_cur][1][lane] = ay;
+      sA[s_cur][2][lane] = az;
+      ++evals;
+    }
+    if (s_cur == 0 && needk) have_k1 = true;
+    ++s_cur;
+    if (s_cur == S) {
+      complete_step();
+      in_step = false;
+    }
+  }
+
+  // rk_step's sums, the error norm and the controller, writing rows that end.
+  __device__ __forceinline__ void complete_step() {
+    const bqp_tableau& TB = c_tab[METHOD];
+    const SecondOrderTableau& SO = c_so[METHOD];
     double yn[6], err[6], comp[6], err3[6], comp3[6];
     {
       const double hb = hs * SO.bsum, he = hs * SO.esum, he3 = hs * SO.e3sum;
@@ -268,34 +365,34 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
       if (bj != 0.0) {
         const double hbj = hs * bj;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) yn[3 + i] = fma(sA[team][j][i][lane], hbj, yn[3 + i]);
+        for (int i = 0; i < 3; ++i) yn[3 + i] = fma(sA[j][i][lane], hbj, yn[3 + i]);
       }
       if (baj != 0.0) {
         const double hba = hh * baj;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) yn[i] = fma(sA[team][j][i][lane], hba, yn[i]);
+        for (int i = 0; i < 3; ++i) yn[i] = fma(sA[j][i][lane], hba, yn[i]);
       }
       if (ej != 0.0) {
         const double hej = hs * ej;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) kahan(err[3 + i], comp[3 + i], hej * sA[team][j][i][lane]);
+        for (int i = 0; i < 3; ++i) kahan(err[3 + i], comp[3 + i], hej * sA[j][i][lane]);
       }
       if (eaj != 0.0) {
         const double hea = hh * eaj;
 #pragma unroll
-        for (int i = 0; i < 3; ++i) kahan(err[i], comp[i], hea * sA[team][j][i][lane]);
+        for (int i = 0; i < 3; ++i) kahan(err[i], comp[i], hea * sA[j][i][lane]);
       }
       if (MT::E3) {
         const double e3j = TB.e3[j], e3aj = SO.e3a[j];
         if (e3j != 0.0) {
           const double h3 = hs * e3j;
 #pragma unroll
-          for (int i = 0; i < 3; ++i) kahan(err3[3 + i], comp3[3 + i], h3 * sA[team][j][i][lane]);
+          for (int i = 0; i < 3; ++i) kahan(err3[3 + i], comp3[3 + i], h3 * sA[j][i][lane]);
         }
         if (e3aj != 0.0) {
           const double h3a = hh * e3aj;
 #pragma unroll
-          for (int i = 0; i < 3; ++i) kahan(err3[i], comp3[i], h3a * sA[team][j][i][lane]);
+          for (int i = 0; i < 3; ++i) kahan(err3[i], comp3[i], h3a * sA[j][i][lane]);
         }
       }
     }
@@ -359,7 +456,7 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
       ++steps;
       rej_run = 0;
       if (MT::FSAL) {
-        for (int i = 0; i < 3; ++i) sA[team][0][i][lane] = sA[team][S - 1][i][lane];
+        for (int i = 0; i < 3; ++i) sA[0][i][lane] = sA[S - 1][i][lane];
         have_k1 = true;
       } else {
         have_k1 = false;
@@ -448,90 +545,56 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
         just_rej = true;
       }
     }
-  };
+  }
+};
+
+// First group (in hand-out positions) of team t of block b: the teams of the
+// blocks are dealt in rounds, round t over blocks 0..C-1 in order, odd rounds
+// in reverse, so each block's teams take ranks b, 2C-1-b, 2C+b, 4C-1-b of the
+// longest-first order and the CUs carry nearly equal work.
+__device__ __forceinline__ unsigned team_first_group(unsigned b, unsigned t, unsigned C) {
+  return t * C + ((t & 1u) ? (C - 1u - b) : b);
+}
+
+template <int METHOD, int PREC>
+__global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const PropParams p) {
+  using MT = MethodTraits<METHOD>;
+  constexpr int S = MT::S;
+  constexpr bool ENCKE = (PREC == 4);
+  static_assert(PREC == 0 || PREC == 4, "team kernel: plain FP64 Cowell or Encke");
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int NT = p.teams;  // 1, 2 or 4 teams per block
+  const bool leader = w < NT;
+  const int team = w;
+
+  __shared__ double sA[TEAM_MAX][S][3][64];   // stage accelerations of each team
+  __shared__ double sYt[TEAM_MAX][6][64];     // inertial stage states (total state for Encke)
+  __shared__ double sTh[TEAM_MAX][64];        // body-frame angle at the stage time
+  __shared__ double sBc[TEAM_MAX][64];        // Cd*A/m of the lanes that store the stage (else 0)
+  __shared__ double sDrag[TEAM_MAX][3][64];   // drag accelerations of the stage
+  __shared__ double sSlot[TEAM_SLOTS][3][64]; // gravity subtree sums of the iteration's work items
+  __shared__ int sAlive[TEAM_MAX];
+  __shared__ int sNext;                       // next work item of the iteration
+
+  GroupLeader<METHOD, PREC> L(p, sA[leader ? team : 0], lane, leader);
+  const unsigned first = leader ? team_first_group(blockIdx.x, (unsigned)team, gridDim.x) : 0u;
+  const unsigned long long dealt_rows = 64ull * (unsigned long long)NT * gridDim.x;
+  const bool drag_on = p.model.drag != DRAG_NONE;
 
   while (true) {
     // ------------------------------------------------------------ A
     if (leader) {
-      if (alive && !in_step) {
-        while (true) {
-          refill();
-          if (wave_ballot(active) == 0ull) {
-            if (exhausted) {
-              alive = false;
-              break;
-            }
-            continue;
-          }
-          stepping = active;
-          const double rem = tseg - t;
-          hs = (h < rem) ? h : rem;
-          lands = (hs == rem);
-          if (stepping) {
-            const int budget = compat ? steps + rejects : steps;
-            if (budget >= p.max_steps || steps + rejects >= 2 * p.max_steps) {
-              finish(ROW_MAX_STEPS);
-              stepping = false;
-            } else if (!compat && hs < hmin && !lands) {
-              finish(ROW_STEP_UNDERFLOW);
-              stepping = false;
-            }
-          }
-          if (wave_ballot(stepping) == 0ull) continue;
-          needk = stepping && !have_k1;
-          s_cur = wave_ballot(needk) != 0ull ? 0 : 1;
-          in_step = true;
-          break;
-        }
-      }
-      if (alive) {
-        const int s = s_cur;
-        st = (s == 0) ? needk : stepping;
-        // stage input (second-order form, as propagate_kernel)
-        double ys[6];
-        const double hh = hs * hs;
-        const double hc = hs * SO.csum[s];
+      L.begin(first, dealt_rows);
+      if (L.alive) {
+        double yt[6];
+        const double tst = L.stage_state(yt);
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          ys[i] = fma(y[3 + i], hc, y[i]);
-          ys[3 + i] = y[3 + i];
-        }
-#pragma unroll
-        for (int j = 0; j < S - 1; ++j) {
-          if (j < s) {
-            const double a = TB.a[s][j];
-            if (a != 0.0) {
-              const double sv = hs * a;
-#pragma unroll
-              for (int i = 0; i < 3; ++i) ys[3 + i] = fma(sA[team][j][i][lane], sv, ys[3 + i]);
-            }
-            const double a2 = SO.a2[s][j];
-            if (a2 != 0.0) {
-              const double sr = hh * a2;
-#pragma unroll
-              for (int i = 0; i < 3; ++i) ys[i] = fma(sA[team][j][i][lane], sr, ys[i]);
-            }
-          }
-        }
-        const double tst = t + TB.c[s] * hs;
-        if constexpr (ENCKE) {
-          double b6[6];
-          two_body(p.mu, el.base, tst - el.t0, b6, el.x);
-#pragma unroll
-          for (int i = 0; i < 6; ++i) sYt[team][i][lane] = b6[i] + ys[i];
-#pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            bst[i] = b6[i];
-            ysr[i] = ys[i];
-          }
-        } else {
-#pragma unroll
-          for (int i = 0; i < 6; ++i) sYt[team][i][lane] = ys[i];
-        }
+        for (int i = 0; i < 6; ++i) sYt[team][i][lane] = yt[i];
         sTh[team][lane] = fma(p.model.omega, tst, p.model.theta0);
-        sBc[team][lane] = st ? bc : 0.0;
+        sBc[team][lane] = L.st ? L.bc : 0.0;
       }
-      if (lane == 0) sAlive[team] = alive ? 1 : 0;
+      if (lane == 0) sAlive[team] = L.alive ? 1 : 0;
     }
     if (w == 0 && lane == 0) sNext = 0;
     __syncthreads();
@@ -618,7 +681,7 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
     }
     __syncthreads();
     // ------------------------------------------------------------ C
-    if (leader && alive) {
+    if (leader && L.alive) {
       // the team's slots combined along the unit tree (in place): units in
       // pairs first when the items were single units, then pairs up to the
       // root
@@ -640,20 +703,120 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
         ax = fma(k, y0, ax);
         ay = fma(k, y1, ay);
         az = fma(k, y2, az);
-        battin(p.mu, bst, ysr, ax, ay, az);
+        battin(p.mu, L.bst, L.ysr, ax, ay, az);
       }
-      if (st) {
-        sA[team][s_cur][0][lane] = ax;
-        sA[team][s_cur][1][lane] = ay;
-        sA[team][s_cur][2][lane] = az;
-        ++evals;
+      L.store_stage(ax, ay, az);
+    }
+  }
+}
+
+// First group (in hand-out positions) of slot k of pair block b over nb
+// blocks: slot 0 takes rank first_group(b) of the first nb (the fixed-G
+// kernel's back-and-forth deal over the CUs), slot 1 the mirror rank
+// 2 nb - 1 - that, so each block pairs an expensive group with a cheap one
+// and CU c holds ranks c, 2C-1-c, 2C+c, 4C-1-c when nb = 2C.
+__device__ __forceinline__ unsigned pair_first_group(unsigned b, unsigned k, unsigned nb, unsigned C) {
+  const unsigned r0 = first_group(b, nb, C);
+  return k == 0u ? r0 : 2u * nb - 1u - r0;
+}
+
+template <int METHOD, int PREC>
+__global__ void __launch_bounds__(64 * PAIR_WAVES) __attribute__((amdgpu_waves_per_eu(BQP_WAVES_PER_EU)))
+propagate_pair_kernel(const PropParams p) {
+  using MT = MethodTraits<METHOD>;
+  constexpr int S = MT::S;
+  constexpr bool ENCKE = (PREC == 4);
+  static_assert(PREC == 0 || PREC == 4, "pair kernel: plain FP64 Cowell or Encke");
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const bool leader = w < PAIR_GROUPS;  // wave g leads group g
+  const int grp = w;
+
+  __shared__ double sA[PAIR_GROUPS][S][3][64];              // stage accelerations of each group
+  __shared__ double sP[PAIR_GROUPS][3][64];                 // body-frame stage positions
+  __shared__ double sPart[PAIR_GROUPS][PAIR_WAVES][3][64];  // gravity of each column share
+  __shared__ int sAlive[PAIR_GROUPS];
+
+  GroupLeader<METHOD, PREC> L(p, sA[leader ? grp : 0], lane, leader);
+  const unsigned first = leader ? pair_first_group(blockIdx.x, (unsigned)grp, gridDim.x, (unsigned)p.deal_cus) : 0u;
+  const unsigned long long dealt_rows = 64ull * PAIR_GROUPS * gridDim.x;
+  // the leader's stage state and frame rotation, kept from phase A to C
+  double yt[6] = {0, 0, 0, 0, 0, 0};
+  double cth = 1.0, sth = 0.0;
+
+  while (true) {
+    // ------------------------------------------------------------ A
+    if (leader) {
+      L.begin(first, dealt_rows);
+      if (L.alive) {
+        const double tst = L.stage_state(yt);
+        sincos(fma(p.model.omega, tst, p.model.theta0), &sth, &cth);
+        sP[grp][0][lane] = fma(cth, yt[0], sth * yt[1]);
+        sP[grp][1][lane] = fma(cth, yt[1], -(sth * yt[0]));
+        sP[grp][2][lane] = yt[2];
       }
-      if (s_cur == 0 && needk) have_k1 = true;
-      ++s_cur;
-      if (s_cur == S) {
-        complete_step();
-        in_step = false;
+      if (lane == 0) sAlive[grp] = L.alive ? 1 : 0;
+    }
+    __syncthreads();
+    // ------------------------------------------------------------ B
+    // two live groups: wave w computes shares 2(w/2) and 2(w/2)+1 of group
+    // w % 2; one: share w of it
+    const int a0 = __builtin_amdgcn_readfirstlane(sAlive[0]);
+    const int a1 = __builtin_amdgcn_readfirstlane(sAlive[1]);
+    if (a0 + a1 == 0) break;
+    {
+      const bool both = (a0 + a1) == 2;
+      const int g = both ? (w & 1) : (a0 ? 0 : 1);
+      const int sh0 = both ? 2 * (w >> 1) : w;
+      const int nsh = both ? 2 : 1;
+      const double xb = sP[g][0][lane], yb = sP[g][1][lane], zb = sP[g][2][lane];
+      for (int k = 0; k < nsh; ++k) {
+        const int share = sh0 + k;
+        double gx, gy, gz;
+        grav_fp64<PAIR_WAVES>(p.field, share, xb, yb, zb, gx, gy, gz);
+        sPart[g][share][0][lane] = gx;
+        sPart[g][share][1][lane] = gy;
+        sPart[g][share][2][lane] = gz;
       }
+    }
+    __syncthreads();
+    // ------------------------------------------------------------ C
+    // the shares in share order, then the rest of accel_eval's arithmetic,
+    // so each acceleration equals the fixed-G kernel's at G = 8
+    if (leader && L.alive) {
+      double gx = sPart[grp][0][0][lane], gy = sPart[grp][0][1][lane], gz = sPart[grp][0][2][lane];
+#pragma unroll
+      for (int s = 1; s < PAIR_WAVES; ++s) {
+        gx += sPart[grp][s][0][lane];
+        gy += sPart[grp][s][1][lane];
+        gz += sPart[grp][s][2][lane];
+      }
+      double ax = fma(cth, gx, -(sth * gy));
+      double ay = fma(sth, gx, cth * gy);
+      double az = gz;
+      if (p.model.drag != DRAG_NONE && L.bc != 0.0) {
+        const double r = sqrt(fma(yt[0], yt[0], fma(yt[1], yt[1], yt[2] * yt[2])));
+        const double rho = density(p.model.drag, r - p.model.r_atm);
+        if (rho > 0.0) {
+          const double vrx = fma(p.model.omega, yt[1], yt[3]);
+          const double vry = fma(-p.model.omega, yt[0], yt[4]);
+          const double vrz = yt[5];
+          const double vr = sqrt(fma(vrx, vrx, fma(vry, vry, vrz * vrz)));
+          const double k = -0.5 * L.bc * rho * vr;
+          ax = fma(k, vrx, ax);
+          ay = fma(k, vry, ay);
+          az = fma(k, vrz, az);
+        }
+      }
+      if constexpr (ENCKE) {
+        const double r2 = fma(yt[0], yt[0], fma(yt[1], yt[1], yt[2] * yt[2]));
+        const double k = p.mu / (r2 * sqrt(r2));
+        ax = fma(k, yt[0], ax);
+        ay = fma(k, yt[1], ay);
+        az = fma(k, yt[2], az);
+        battin(p.mu, L.bst, L.ysr, ax, ay, az);
+      }
+      L.store_stage(ax, ay, az);
     }
   }
 }

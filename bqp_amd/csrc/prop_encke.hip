@@ -23,4 +23,13 @@ prop_fn pick_team_encke(int method) {
     default: return nullptr;
   }
 }
+
+prop_fn pick_pair_encke(int method) {
+  switch (method) {
+    case BQP_METHOD_DOPRI5: return propagate_pair_kernel<BQP_METHOD_DOPRI5, 4>;
+    case BQP_METHOD_DOP853: return propagate_pair_kernel<BQP_METHOD_DOP853, 4>;
+    case BQP_METHOD_RK87: return propagate_pair_kernel<BQP_METHOD_RK87, 4>;
+    default: return nullptr;
+  }
+}
 }  // namespace bqp

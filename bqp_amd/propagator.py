@@ -150,10 +150,12 @@ class Propagator:
         """Launch shape: waves per trajectory group G (0 = auto), HIP-graph
         replay, persistent grid size (0 = fill the device), hand-out order
         (0 input order, 1 longest first by the learned cost model, 2 longest
-        first by revolutions), and the team kernel's groups per workgroup
-        (0 = the fixed-G kernel; 1, 2 or 4 = the team kernel, whose results
-        do not depend on the launch shape or the batch, with G = 0, FP64 or
-        Encke, fields of degree >= 30)."""
+        first by revolutions), and the groups per workgroup (0 = the
+        fixed-G kernel; 1, 2 or 4 with G = 0 = the team kernel, whose
+        results do not depend on the launch shape or the batch, FP64 or
+        Encke, fields of degree >= 30; 2 with G = 8 = the pair kernel, two
+        groups per 8-wave workgroup whose results equal G = 8's, FP64 or
+        Encke)."""
         L = nat.Launch(waves_per_group, use_graph, blocks, sort_by_cost, teams_per_block)
         self._check(self._lib.bqp_set_launch(self._ctx, C.byref(L)))
         self.launch = {"waves_per_group": waves_per_group, "use_graph": use_graph, "blocks": blocks,

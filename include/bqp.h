@@ -133,7 +133,11 @@ typedef struct bqp_launch {
                                per CU carrying that many 64-trajectory groups, whose waves take the
                                gravity work of whichever groups are still running; its results
                                do not depend on grid, group width or batch composition (bit for
-                               bit), but it measures slower on the 65,536-row headline batch */
+                               bit), but it measures slower on the 65,536-row headline batch.
+                               2 with waves_per_group = 8 (FP64 or Encke): the pair kernel, two
+                               groups per 8-wave workgroup, a group's gravity in the 8 column
+                               shares of G = 8 so that it runs on all 8 waves once its partner is
+                               done; results equal waves_per_group = 8 (bit for bit) */
 } bqp_launch;
 
 /* Cells of the learned cost model: BQP_COST_ALT_BINS log-spaced bins of the

@@ -1,0 +1,120 @@
+"""GPU tests of the pair kernel (bqp_team.hpp): two 64-trajectory groups per
+8-wave workgroup, each group's gravity in the eight column shares of the
+fixed-G kernel at G = 8, so that a group whose partner is done runs on all
+eight waves.  Taken by FP64 and Encke calls with waves_per_group = 8 and
+teams_per_block = 2.
+
+* Every result -- final states, step, evaluation and reject counts, status
+  -- equals the fixed-G kernel's at G = 8 bit for bit: RK8(7)13M with
+  Vallado drag on ragged durations with more rows than the dealt round
+  (queue refills), DOPRI5 (FSAL reuse), DOP853, the compat controller,
+  Encke, sampled output, failing rows, and a batch smaller than one pair.
+* against the oracle to the usual bars.
+"""
+import numpy as np
+import pytest
+
+import bqp_amd
+from bqp_amd import inputs
+from gpu_util import announce, oracle_pair, pos_err
+
+pytestmark = pytest.mark.gpu
+
+DAY = 86400.0
+PAIR = {"waves_per_group": 8, "teams_per_block": 2}
+FIXED8 = {"waves_per_group": 8, "teams_per_block": 0}
+
+
+def _both(field, y0, t0, tf, bc=None, **kw):
+    out = []
+    for launch in (PAIR, FIXED8):
+        with bqp_amd.Propagator(field, launch=launch, **kw) as p:
+            y, s, info = p.propagate_batch_final(y0, t0, tf, bc)
+        out.append((y, s, info))
+    return out
+
+
+def _assert_same(a, b):
+    (ya, sa, ia), (yb, sb, ib) = a, b
+    assert np.array_equal(sa, sb)
+    assert np.array_equal(ya, yb, equal_nan=True)
+    for k in ("steps_per_row", "evals_per_row"):
+        assert np.array_equal(ia[k], ib[k]), k
+
+
+@pytest.mark.parametrize("method,control,nmax,drag", [("rk87", "scaled", 40, "vallado"),
+                                                      ("dopri5", "scaled", 20, "none"),
+                                                      ("dop853", "scaled", 30, "vallado"),
+                                                      ("dopri5", "compat", 12, "none")])
+def test_pair_equals_fixed_g8(method, control, nmax, drag):
+    n = 70_000  # more than one dealt round on the device: queue refills
+    w = inputs.workload("w1", n=n, seed=41)
+    rng = np.random.default_rng(41)
+    tf = rng.uniform(600.0, 3 * 3600.0, n)
+    announce(f"pair kernel vs fixed G=8: {method}/{control}, N={nmax}, drag={drag}, {n} rows, ragged 10 min-3 h")
+    field = bqp_amd.kaula(nmax)
+    bc = w["bc"] if drag != "none" else None
+    a, b = _both(field, w["y0"], 0.0, tf, bc, method=method, control=control, rtol=1e-11, drag=drag)
+    assert np.all(a[1] == 0)
+    _assert_same(a, b)
+
+
+def test_pair_encke_and_sampled_equal_fixed_g8():
+    n = 3000
+    w = inputs.workload("w1", n=n, seed=42)
+    field = bqp_amd.kaula(30)
+    announce("pair kernel vs fixed G=8: Encke RK8(7)13M 7 h with drag; sampled Cowell at 6 times")
+    a, b = _both(field, w["y0"], 0.0, np.full(n, 7 * 3600.0), w["bc"], method="rk87", control="absolute",
+                 eps=1e-6, drag="vallado", formulation="encke")
+    assert np.all(a[1] == 0)
+    _assert_same(a, b)
+    te = np.array([0.0, 600.0, 1800.0, 3600.0, 5000.0, 9000.0])
+    res = []
+    for launch in (PAIR, FIXED8):
+        with bqp_amd.Propagator(field, method="rk87", rtol=1e-12, launch=launch) as p:
+            res.append(p.propagate_batch_sampled(w["y0"][:500], 0.0, te))
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+
+
+def test_pair_failing_rows_and_tiny_batches():
+    w = inputs.workload("w1", n=200, seed=43)
+    field = bqp_amd.kaula(20)
+    y0 = w["y0"].copy()
+    y0[3, 2] = np.nan          # invalid input
+    y0[70, :3] *= 0.5          # below the ground: a status, NaN-filled
+    tf = np.full(200, 3600.0)
+    tf[5] = 0.0                # zero span
+    announce("pair kernel vs fixed G=8: invalid rows, zero span, max-steps, batches of 1, 63, 65 rows")
+    a, b = _both(field, y0, 0.0, tf, method="rk87", rtol=1e-11, r_min=6378137.0, max_steps=300)
+    _assert_same(a, b)
+    assert a[1][3] != 0 and a[1][70] != 0 and a[1][5] == 0
+    for n in (1, 63, 65):
+        a, b = _both(field, w["y0"][:n], 0.0, np.full(n, 1800.0), method="rk87", rtol=1e-11)
+        assert np.all(a[1] == 0)
+        _assert_same(a, b)
+
+
+def test_pair_kernel_vs_oracle(orc):
+    n = 128
+    w = inputs.workload("w1", n=n, seed=44)
+    field = bqp_amd.kaula(40)
+    announce("pair kernel vs oracle: 128 rows, N=40, Vallado drag, 6 h")
+    with bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado", launch=PAIR) as p:
+        yg, sg, _ = p.propagate_batch_final(w["y0"], 0.0, np.full(n, 6 * 3600.0), w["bc"])
+    _, model, integ = oracle_pair(orc, field, method="rk87", rtol=1e-12, drag="vallado")
+    yo, so, _, _ = orc.propagate_batch(model, integ, w["y0"], 0.0, 6 * 3600.0, w["bc"])
+    assert np.all(sg == 0) and np.all(so == 0)
+    err = pos_err(yg, yo)
+    announce(f"  pair vs oracle max {err.max():.3e} m")
+    assert err.max() < 1e-2
+
+
+def test_launch_combinations_are_checked():
+    field = bqp_amd.kaula(8)
+    with bqp_amd.Propagator(field, method="rk87", rtol=1e-11) as p:
+        p.set_launch(**PAIR)
+        for bad in ({"waves_per_group": 4, "teams_per_block": 2}, {"waves_per_group": 8, "teams_per_block": 4},
+                    {"waves_per_group": 16, "teams_per_block": 1}):
+            with pytest.raises(Exception, match="teams_per_block"):
+                p.set_launch(**bad)
