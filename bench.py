@@ -493,11 +493,6 @@ def run_population(args) -> dict:
         ["fp64", "surrogate_bf16", "surrogate_fp8"]
     out, finals, sur = {}, {}, None
     prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado")
-    # one 4,096-candidate generation is 64 groups: the team kernel with one
-    # group per workgroup puts all 16 waves of a CU on it (measured 17%
-    # faster than 16 waves per group in the fixed-G kernel on the first
-    # generation, whose longest arc takes 1.2 million evaluations)
-    prop.set_launch(teams_per_block=1)
     for form in forms:
         if form.startswith("surrogate"):
             if sur is None:
