@@ -478,9 +478,11 @@ def run_population(args) -> dict:
     """Population evaluation: 32 NSGA-II generations x 4,096 candidates
     (131,072 arcs) over seeded LEO departure states with dv U(-50, 50) m/s
     per axis and coast U(0.5, 10) d, degree/order 50 + Vallado drag,
-    RK8(7)13M rtol 1e-12.  Each generation is one ragged device batch; the
-    field and the surrogate stay resident between generations.  Forms: plain
-    FP64, then the checked surrogate with a BF16 and with an FP8 network."""
+    RK8(7)13M rtol 1e-12, ground event 100 km above the WGS84 ellipsoid (a
+    candidate that re-enters ends there and loses; its arc is not counted in
+    the orbit-days).  Each generation is one ragged device batch; the field
+    and the surrogate stay resident between generations.  Forms: plain FP64,
+    then the checked surrogate with a BF16 and with an FP8 network."""
     import bqp_amd
     from bqp_amd import inputs
     from bqp_amd.population import DepartureProblem, Nsga2
@@ -492,7 +494,11 @@ def run_population(args) -> dict:
     forms = [f for f in args.forms.split(",") if f] if args.forms != "fp64,split,surrogate,encke" else \
         ["fp64", "surrogate_bf16", "surrogate_fp8"]
     out, finals, sur = {}, {}, None
-    prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado")
+    # the reference's ground event: a candidate that re-enters fails there
+    # (two_phase_transfer_rs/src/evaluate.rs:1044-1060) and loses
+    prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado",
+                              r_min=bqp_amd.R_EARTH + bqp_amd.GROUND_ALTITUDE_M,
+                              ground_flattening=bqp_amd.WGS84_FLATTENING)
     for form in forms:
         if form.startswith("surrogate"):
             if sur is None:

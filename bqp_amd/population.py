@@ -252,7 +252,10 @@ class DepartureProblem:
     [t_min, t_max] days).  Objectives (minimise): |dv| (m/s) and
     | |r(T)| - r_target | (km) -- reach a target radius at the end of the
     coast.  Each generation is one ragged batch on the GPU (rows = candidates,
-    per-row tf = T, per-row drag coefficient of the departure state).
+    per-row tf = T, per-row drag coefficient of the departure state).  Rows
+    that fail (with the reference's ground event on the propagator, a
+    candidate that re-enters: FinalPropagationFailure::Ground,
+    two_phase_transfer_rs/src/evaluate.rs:1044-1060) lose.
     """
 
     def __init__(self, prop, states: np.ndarray, bc: np.ndarray | None, r_target: float,
@@ -293,7 +296,9 @@ class DepartureProblem:
         yf, status, info = self.prop.propagate_batch_final(y0, 0.0, tf, bc)
         self.device_s += time.perf_counter() - t
         self.calls += 1
-        self.orbit_days += float(np.sum(tf) / DAY)
+        # only arcs that reach their coast time count: a row that ends early
+        # (the ground event) is not charged its requested span
+        self.orbit_days += float(np.sum(tf[status == 0]) / DAY)
         self.evals += int(info["evals"])
         self.last = (yf, status, info)
         dv = np.linalg.norm(X[:, 1:4], axis=1)

@@ -19,9 +19,9 @@ def product(field, **kw):
 
 
 def oracle_pair(orc, field, method="rk87", control="scaled", drag="none", rtol=1e-12, eps=1e-8, h_max=0.0,
-                r_min=0.0, max_steps=500000):
+                r_min=0.0, max_steps=500000, ground_flattening=0.0):
     f = orc.Field(field.cbar, field.sbar, field.nmax, field.mu, field.radius, mmax=field.mmax)
-    model = orc.make_model(f, drag=drag, r_min=r_min)
+    model = orc.make_model(f, drag=drag, r_min=r_min, ground_flattening=ground_flattening)
     integ = orc.make_integ(method=method, control=control, rtol=rtol, eps=eps, h_max=h_max, max_steps=max_steps,
                            canonical=(field.radius, field.mu))
     return f, model, integ
