@@ -1051,9 +1051,13 @@ __device__ __forceinline__ void grav_unit_inertial(const FieldView& f, const Uni
 
 // Vallado (2013) Table 8-4: band base altitude (km), nominal density
 // (kg/m^3), scale height (km).
-static __constant__ double c_vallado_h0[28] = {0,   25,  30,  40,  50,  60,  70,  80,  90,  100,
-                                        110, 120, 130, 140, 150, 180, 200, 250, 300, 350,
-                                        400, 450, 500, 600, 700, 800, 900, 1000};
+constexpr int VALLADO_BANDS = 28;
+constexpr int kValladoH0Km[VALLADO_BANDS] = {0,   25,  30,  40,  50,  60,  70,  80,  90,  100,
+                                             110, 120, 130, 140, 150, 180, 200, 250, 300, 350,
+                                             400, 450, 500, 600, 700, 800, 900, 1000};
+static __constant__ double c_vallado_h0[VALLADO_BANDS] = {0,   25,  30,  40,  50,  60,  70,  80,  90,  100,
+                                                          110, 120, 130, 140, 150, 180, 200, 250, 300, 350,
+                                                          400, 450, 500, 600, 700, 800, 900, 1000};
 static __constant__ double c_vallado_rho0[28] = {
     1.225,     3.899e-2,  1.774e-2,  3.972e-3,  1.057e-3,  3.206e-4,  8.770e-5,
     1.905e-5,  3.396e-6,  5.297e-7,  9.661e-8,  2.438e-8,  8.484e-9,  3.845e-9,
@@ -1066,20 +1070,42 @@ static __constant__ double c_vallado_H[28] = {7.249,  6.349,  6.682,  7.554,  8.
 
 enum { DRAG_NONE = 0, DRAG_VALLADO = 1, DRAG_COMPAT = 2 };
 
+// Every band base is a multiple of 5 km, so the band of an altitude in
+// [0, 1000) km is a function of floor(h / 5 km): entry k is the last band
+// whose base is at most 5k km.
+struct ValladoBand5 {
+  unsigned char band[200];
+};
+constexpr ValladoBand5 make_vallado_band5() {
+  ValladoBand5 t{};
+  int i = 0;
+  for (int k = 0; k < 200; ++k) {
+    while (i + 1 < VALLADO_BANDS && kValladoH0Km[i + 1] <= 5 * k) ++i;
+    t.band[k] = (unsigned char)i;
+  }
+  return t;
+}
+static __constant__ ValladoBand5 c_vallado_band5 = make_vallado_band5();
+
+// Band of altitude hk (km): the last band whose base is at most hk (band 0
+// below 0 km and for NaN; the top band from 1000 km on).  floor(hk / 5) is
+// taken from a truncated product and corrected by exact comparisons with 5k
+// (multiples of 5 up to 1000 are exact doubles), so it is exact.
+__device__ __forceinline__ int vallado_band(double hk) {
+  if (hk >= 1000.0) return VALLADO_BANDS - 1;
+  if (!(hk >= 0.0)) return 0;
+  int k = (int)(hk * 0.2);
+  if (k > 0 && 5.0 * k > hk) --k;
+  if (k < 199 && 5.0 * (k + 1) <= hk) ++k;
+  return c_vallado_band5.band[k];
+}
+
 __device__ __forceinline__ double density(int model, double h_m) {
   const double hk = h_m * 1e-3;
   if (model == DRAG_VALLADO) {
-    // band search with wave-uniform table reads (scalar loads + selects)
-    double b0 = c_vallado_h0[0], r0 = c_vallado_rho0[0], H = c_vallado_H[0];
-#pragma unroll 4
-    for (int i = 1; i < 28; ++i) {
-      const bool up = hk >= c_vallado_h0[i];
-      b0 = up ? c_vallado_h0[i] : b0;
-      r0 = up ? c_vallado_rho0[i] : r0;
-      H = up ? c_vallado_H[i] : H;
-    }
     if (hk < 0.0) return c_vallado_rho0[0];
-    return r0 * exp(-(hk - b0) / H);
+    const int b = vallado_band(hk);
+    return c_vallado_rho0[b] * exp(-(hk - c_vallado_h0[b]) / c_vallado_H[b]);
   }
   if (model == DRAG_COMPAT) {
     if (hk <= 0.0) return 1.225;
