@@ -58,6 +58,12 @@ __device__ __forceinline__ unsigned first_group(unsigned b, unsigned nb, unsigne
   return b;
 }
 
+// Stage sums read the earlier stages j < s in chunks of four; chunk k of a
+// stage row holds {a_sj, (A.A)_sj} for j = 4k .. 4k+3, interleaved, so one
+// chunk is one 64-byte scalar load.
+constexpr int STAGE_CHUNK = 4;
+constexpr int STAGE_CHUNKS = (BQP_MAX_STAGES - 1 + STAGE_CHUNK - 1) / STAGE_CHUNK;
+
 // Second-order rearrangement of each tableau, computed on the host
 // (make_second_order) and copied to the device when a context is created.
 struct SecondOrderTableau {
@@ -67,6 +73,7 @@ struct SecondOrderTableau {
   double ea[BQP_MAX_STAGES];                    // (e.A)_j
   double e3a[BQP_MAX_STAGES];                   // (e3.A)_j
   double bsum, esum, e3sum;
+  double sc[BQP_MAX_STAGES][STAGE_CHUNKS][2 * STAGE_CHUNK];  // packed stage rows (zero for j >= s)
 };
 static __constant__ SecondOrderTableau c_so[3];
 
@@ -96,6 +103,11 @@ inline void make_second_order(const bqp_tableau& t, SecondOrderTableau& o) {
   o.bsum = (double)bs;
   o.esum = (double)es;
   o.e3sum = (double)e3s;
+  for (int i = 0; i < S; ++i)
+    for (int j = 0; j < i && j < BQP_MAX_STAGES - 1; ++j) {
+      o.sc[i][j / STAGE_CHUNK][2 * (j % STAGE_CHUNK)] = t.a[i][j];
+      o.sc[i][j / STAGE_CHUNK][2 * (j % STAGE_CHUNK) + 1] = o.a2[i][j];
+    }
 }
 
 template <int METHOD>
@@ -118,6 +130,62 @@ struct MethodTraits<BQP_METHOD_RK87> {
   static constexpr bool FSAL = false;
   static constexpr bool E3 = false;
 };
+
+// Stage state of stage s in the second-order form (the rearranged rk_step
+// stage sums, LY/src/odesolve/solver.rs:1965-2173):
+//   R_s = r + h c^_s v + h^2 sum_{j<s} (A.A)_sj A_j,   V_s = v + h sum_{j<s} a_sj A_j,
+// each component summed in ascending j.  The terms of a chunk of four
+// earlier stages are read together (one scalar load of the coefficients,
+// twelve LDS reads) and added without branches: a zero coefficient adds an
+// exact zero, so for finite stage accelerations the sums equal those that
+// skip zero coefficients bit for bit, and slots j >= s (stale) are replaced
+// by zero before use.  A non-finite stage acceleration makes the step's
+// solution non-finite either way.  This keeps the stage sums off the chain
+// of dependent scalar loads and branches that one term at a time costs.
+template <int METHOD>
+__device__ __forceinline__ void stage_sums(int s, double hs, const double* y, const double (*sA)[3][64], int lane,
+                                           double* ys) {
+  constexpr int S = MethodTraits<METHOD>::S;
+  constexpr int NCH = (S - 1 + STAGE_CHUNK - 1) / STAGE_CHUNK;
+  const SecondOrderTableau& SO = c_so[METHOD];
+  const double hh = hs * hs;
+  const double hc = hs * SO.csum[s];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    ys[i] = fma(y[3 + i], hc, y[i]);
+    ys[3 + i] = y[3 + i];
+  }
+#pragma unroll
+  for (int k = 0; k < NCH; ++k) {
+    if (STAGE_CHUNK * k < s) {
+      const double* cf = SO.sc[s][k];
+      double A[STAGE_CHUNK][3];
+#pragma unroll
+      for (int m = 0; m < STAGE_CHUNK; ++m) {
+        const int j = STAGE_CHUNK * k + m;
+        if (j < S - 1) {
+          const bool use = j < s;
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            const double v = sA[j][i][lane];
+            A[m][i] = use ? v : 0.0;
+          }
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < STAGE_CHUNK; ++m) {
+        const int j = STAGE_CHUNK * k + m;
+        if (j < S - 1) {
+          const double sv = hs * cf[2 * m], sr = hh * cf[2 * m + 1];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) ys[3 + i] = fma(A[m][i], sv, ys[3 + i]);
+#pragma unroll
+          for (int i = 0; i < 3; ++i) ys[i] = fma(A[m][i], sr, ys[i]);
+        }
+      }
+    }
+  }
+}
 
 enum { CONTROL_SCALED = 0, CONTROL_ABSOLUTE = 1, CONTROL_COMPAT = 2 };
 enum {
@@ -731,29 +799,7 @@ propagate_kernel(const PropParams p) {
       const bool st = (s == 0) ? needk : stepping;
       if (s == 0 && wave_ballot(needk) == 0ull) continue;
       double ys[6];
-      const double hc = hs * SO.csum[s];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        ys[i] = fma(y[3 + i], hc, y[i]);
-        ys[3 + i] = y[3 + i];
-      }
-#pragma unroll
-      for (int j = 0; j < S - 1; ++j) {
-        if (j < s) {
-          const double a = T.a[s][j];
-          if (a != 0.0) {
-            const double sv = hs * a;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) ys[3 + i] = fma(sA[j][i][lane], sv, ys[3 + i]);
-          }
-          const double a2 = SO.a2[s][j];
-          if (a2 != 0.0) {
-            const double sr = hh * a2;
-#pragma unroll
-            for (int i = 0; i < 3; ++i) ys[i] = fma(sA[j][i][lane], sr, ys[i]);
-          }
-        }
-      }
+      stage_sums<METHOD>(s, hs, y, sA, lane, ys);
       double as[3];
       if (SUR) sl.check_now = p.sur_check_every > 0 && (evals % p.sur_check_every) == 0;
       accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, st, bc, t + T.c[s] * hs, ys, as, sX, sO, sFlag, sl, el, sP);

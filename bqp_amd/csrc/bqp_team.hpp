@@ -270,34 +270,10 @@ struct GroupLeader {
   // order form, as propagate_kernel) into yt; returns the stage time.
   __device__ __forceinline__ double stage_state(double* yt) {
     const bqp_tableau& TB = c_tab[METHOD];
-    const SecondOrderTableau& SO = c_so[METHOD];
     const int s = s_cur;
     st = (s == 0) ? needk : stepping;
     double ys[6];
-    const double hh = hs * hs;
-    const double hc = hs * SO.csum[s];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      ys[i] = fma(y[3 + i], hc, y[i]);
-      ys[3 + i] = y[3 + i];
-    }
-#pragma unroll
-    for (int j = 0; j < S - 1; ++j) {
-      if (j < s) {
-        const double a = TB.a[s][j];
-        if (a != 0.0) {
-          const double sv = hs * a;
-#pragma unroll
-          for (int i = 0; i < 3; ++i) ys[3 + i] = fma(sA[j][i][lane], sv, ys[3 + i]);
-        }
-        const double a2 = SO.a2[s][j];
-        if (a2 != 0.0) {
-          const double sr = hh * a2;
-#pragma unroll
-          for (int i = 0; i < 3; ++i) ys[i] = fma(sA[j][i][lane], sr, ys[i]);
-        }
-      }
-    }
+    stage_sums<METHOD>(s, hs, y, sA, lane, ys);
     const double tst = t + TB.c[s] * hs;
     if constexpr (ENCKE) {
       double b6[6];
