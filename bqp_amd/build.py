@@ -2,12 +2,15 @@
 
 Flags: ``-ffp-contract=off`` so only the explicit fma() calls fuse (the CPU
 restatement uses the same rule), and the packed-FP32 target feature disabled.
+The second-order tableau constants (csrc/second_order_tables.h) are
+regenerated by the host program csrc/gen_second_order.cpp first.
 """
 from __future__ import annotations
 
 import os
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRCS = [os.path.join(HERE, "csrc", f) for f in
@@ -28,9 +31,24 @@ FLAGS = [
 ]
 
 
+def gen_second_order() -> str:
+    """Regenerate csrc/second_order_tables.h: the second-order tableau
+    rearrangement (csrc/second_order.h), formed in long double on the host by
+    csrc/gen_second_order.cpp and printed as hexadecimal literals."""
+    src = os.path.join(HERE, "csrc", "gen_second_order.cpp")
+    out = os.path.join(HERE, "csrc", "second_order_tables.h")
+    with tempfile.TemporaryDirectory() as d:
+        exe = os.path.join(d, "gen_second_order")
+        subprocess.run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-o", exe, src], check=True)
+        subprocess.run([exe, out], check=True)
+    return out
+
+
 def build(verbose: bool = False) -> str:
     """Compile libbqp.so (always recompiles); returns its path.  The
-    translation units compile in parallel, then one link."""
+    tableau constants are regenerated first; the translation units compile
+    in parallel, then one link."""
+    gen_second_order()
     objs = [os.path.join(HERE, "csrc", os.path.basename(src) + ".o") for src in SRCS]
     compile_flags = [f for f in FLAGS if f != "-shared"]
     procs = [subprocess.Popen([HIPCC, *compile_flags, "-c", "-o", obj, src], stdout=subprocess.PIPE,

@@ -817,20 +817,16 @@ int32_t bqp_create(const bqp_config* cfg, bqp_ctx** out) {
   // lookup allocates nothing; a failure only disables that entry point)
   (void)noalloc_base(dev);
   {
-    static const bqp_tableau tabs[3] = {BQP_DOPRI5_TABLEAU, BQP_DOP853_TABLEAU, BQP_RK87_TABLEAU};
-    SecondOrderTableau so[3];
-    for (int m = 0; m < 3; ++m) make_second_order(tabs[m], so[m]);
     double sect[SECT_MAX];
     float sectf[SECT_MAX];
     sect[0] = 0.0;
     for (int j = 1; j < SECT_MAX; ++j) sect[j] = (j == 1) ? std::sqrt(3.0) : std::sqrt((2.0 * j + 1.0) / (2.0 * j));
     for (int j = 0; j < SECT_MAX; ++j) sectf[j] = (float)sect[j];
-    if (hipMemcpyToSymbol(HIP_SYMBOL(c_so), so, sizeof(so)) != hipSuccess ||
-        hipMemcpyToSymbol(HIP_SYMBOL(c_sect), sect, sizeof(sect)) != hipSuccess ||
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_sect), sect, sizeof(sect)) != hipSuccess ||
         hipMemcpyToSymbol(HIP_SYMBOL(c_sectf), sectf, sizeof(sectf)) != hipSuccess ||
-        upload_fp64(so, sect, sectf) != hipSuccess || upload_split(so, sect, sectf) != hipSuccess ||
-        upload_sur(so, sect, sectf) != hipSuccess || upload_sur_fp8(so, sect, sectf) != hipSuccess ||
-        upload_encke(so, sect, sectf) != hipSuccess) {
+        upload_fp64(sect, sectf) != hipSuccess || upload_split(sect, sectf) != hipSuccess ||
+        upload_sur(sect, sectf) != hipSuccess || upload_sur_fp8(sect, sectf) != hipSuccess ||
+        upload_encke(sect, sectf) != hipSuccess) {
       bqp_destroy(c);
       return BQP_E_DEVICE;
     }

@@ -34,6 +34,8 @@
 
 #include "bqp_device.hpp"
 #include "mlp_device.hpp"
+#include "second_order.h"
+#include "second_order_tables.h"
 
 #ifndef BQP_WAVES_PER_EU
 #define BQP_WAVES_PER_EU 4
@@ -41,7 +43,7 @@
 
 namespace bqp {
 
-static __constant__ bqp_tableau c_tab[3] = {BQP_DOPRI5_TABLEAU, BQP_DOP853_TABLEAU, BQP_RK87_TABLEAU};
+static constexpr bqp_tableau c_tab[3] = {BQP_DOPRI5_TABLEAU, BQP_DOP853_TABLEAU, BQP_RK87_TABLEAU};
 
 // Hand-out position (in groups) of block b's first group.  Every lane starts
 // free, so the first round of groups is dealt by block index instead of
@@ -58,57 +60,11 @@ __device__ __forceinline__ unsigned first_group(unsigned b, unsigned nb, unsigne
   return b;
 }
 
-// Stage sums read the earlier stages j < s in chunks of four; chunk k of a
-// stage row holds {a_sj, (A.A)_sj} for j = 4k .. 4k+3, interleaved, so one
-// chunk is one 64-byte scalar load.
-constexpr int STAGE_CHUNK = 4;
-constexpr int STAGE_CHUNKS = (BQP_MAX_STAGES - 1 + STAGE_CHUNK - 1) / STAGE_CHUNK;
-
-// Second-order rearrangement of each tableau, computed on the host
-// (make_second_order) and copied to the device when a context is created.
-struct SecondOrderTableau {
-  double csum[BQP_MAX_STAGES];                  // sum_j a_sj
-  double a2[BQP_MAX_STAGES][BQP_MAX_STAGES];    // (A.A)_sj
-  double ba[BQP_MAX_STAGES];                    // (b.A)_j
-  double ea[BQP_MAX_STAGES];                    // (e.A)_j
-  double e3a[BQP_MAX_STAGES];                   // (e3.A)_j
-  double bsum, esum, e3sum;
-  double sc[BQP_MAX_STAGES][STAGE_CHUNKS][2 * STAGE_CHUNK];  // packed stage rows (zero for j >= s)
-};
-static __constant__ SecondOrderTableau c_so[3];
-
-inline void make_second_order(const bqp_tableau& t, SecondOrderTableau& o) {
-  const int S = t.stages;
-  o = SecondOrderTableau{};
-  long double bs = 0, es = 0, e3s = 0;
-  for (int i = 0; i < S; ++i) {
-    long double cs = 0, ba = 0, ea = 0, e3a = 0;
-    for (int j = 0; j < S; ++j) {
-      cs += (long double)t.a[i][j];
-      long double a2 = 0;
-      for (int k = 0; k < S; ++k) a2 += (long double)t.a[i][k] * (long double)t.a[k][j];
-      o.a2[i][j] = (double)a2;
-      ba += (long double)t.b[j] * (long double)t.a[j][i];
-      ea += (long double)t.e[j] * (long double)t.a[j][i];
-      e3a += (long double)t.e3[j] * (long double)t.a[j][i];
-    }
-    o.csum[i] = (double)cs;
-    o.ba[i] = (double)ba;
-    o.ea[i] = (double)ea;
-    o.e3a[i] = (double)e3a;
-    bs += t.b[i];
-    es += t.e[i];
-    e3s += t.e3[i];
-  }
-  o.bsum = (double)bs;
-  o.esum = (double)es;
-  o.e3sum = (double)e3s;
-  for (int i = 0; i < S; ++i)
-    for (int j = 0; j < i && j < BQP_MAX_STAGES - 1; ++j) {
-      o.sc[i][j / STAGE_CHUNK][2 * (j % STAGE_CHUNK)] = t.a[i][j];
-      o.sc[i][j / STAGE_CHUNK][2 * (j % STAGE_CHUNK) + 1] = o.a2[i][j];
-    }
-}
+// Second-order rearrangement of each tableau (second_order.h), generated on
+// the host at build time (gen_second_order.cpp -> second_order_tables.h) and
+// compiled in as constants: coefficients read at compile-time indices fold
+// into the instructions, and zero weights drop out of the code.
+static constexpr SecondOrderTableau c_so[3] = BQP_SECOND_ORDER_TABLES;
 
 template <int METHOD>
 struct MethodTraits;

@@ -3,7 +3,8 @@
 // Each precision form (FP64, split, surrogate) instantiates its kernels in a
 // translation unit of its own (prop_fp64.hip, prop_split.hip, prop_sur.hip, prop_sur_fp8.hip, prop_encke.hip)
 // so the forms compile in parallel.  A unit has its own copy of the
-// __constant__ tables, which bqp_create fills through its upload function.
+// __constant__ tables of sectoral factors, which bqp_create fills through its
+// upload function (the tableaus are compiled-in constants).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,11 +29,11 @@ prop_fn pick_team_encke(int method);
 prop_fn pick_pair_fp64(int method);
 prop_fn pick_pair_encke(int method);
 
-hipError_t upload_fp64(const SecondOrderTableau* so, const double* sect, const float* sectf);
-hipError_t upload_split(const SecondOrderTableau* so, const double* sect, const float* sectf);
-hipError_t upload_sur(const SecondOrderTableau* so, const double* sect, const float* sectf);
-hipError_t upload_sur_fp8(const SecondOrderTableau* so, const double* sect, const float* sectf);
-hipError_t upload_encke(const SecondOrderTableau* so, const double* sect, const float* sectf);
+hipError_t upload_fp64(const double* sect, const float* sectf);
+hipError_t upload_split(const double* sect, const float* sectf);
+hipError_t upload_sur(const double* sect, const float* sectf);
+hipError_t upload_sur_fp8(const double* sect, const float* sectf);
+hipError_t upload_encke(const double* sect, const float* sectf);
 
 // Body of a form's unit: kernel selection over the four group widths and
 // the constant upload.
@@ -50,9 +51,8 @@ hipError_t upload_encke(const SecondOrderTableau* so, const double* sect, const 
     }                                                                                          \
   }                                                                                            \
   }                                                                                            \
-  hipError_t UPLOAD(const SecondOrderTableau* so, const double* sect, const float* sectf) {    \
-    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_so), so, 3 * sizeof(SecondOrderTableau));    \
-    if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_sect), sect, SECT_MAX * sizeof(double)); \
+  hipError_t UPLOAD(const double* sect, const float* sectf) {                                  \
+    hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(c_sect), sect, SECT_MAX * sizeof(double));     \
     if (e == hipSuccess) e = hipMemcpyToSymbol(HIP_SYMBOL(c_sectf), sectf, SECT_MAX * sizeof(float)); \
     return e;                                                                                  \
   }
