@@ -87,12 +87,39 @@ struct MethodTraits<BQP_METHOD_RK87> {
   static constexpr bool E3 = false;
 };
 
+// Stage rows of a method staged in LDS at kernel start: row s holds c^_s,
+// c_s and the packed chunks of SecondOrderTableau::sc.  The scalar cache is
+// streamed through by the gravity table, so these per-stage reads would
+// otherwise go to L2 on every evaluation; from LDS they arrive together with
+// the stage accelerations of the same chunk.
+constexpr int STAGE_ROW = 2 + STAGE_CHUNKS * 2 * STAGE_CHUNK;
+
+// Fills rows[s][*] for every stage; called by all threads of the workgroup
+// (thread tid of nthreads) before a barrier.
+template <int METHOD>
+__device__ __forceinline__ void stage_rows_to_lds(double (*rows)[STAGE_ROW], int tid, int nthreads) {
+  constexpr int S = MethodTraits<METHOD>::S;
+  for (int k = tid; k < S * STAGE_ROW; k += nthreads) {
+    const int st = k / STAGE_ROW, i = k - st * STAGE_ROW;
+    double v;
+    if (i == 0) {
+      v = c_so[METHOD].csum[st];
+    } else if (i == 1) {
+      v = c_tab[METHOD].c[st];
+    } else {
+      const int q = i - 2;
+      v = c_so[METHOD].sc[st][q / (2 * STAGE_CHUNK)][q % (2 * STAGE_CHUNK)];
+    }
+    rows[st][i] = v;
+  }
+}
+
 // Stage state of stage s in the second-order form (the rearranged rk_step
 // stage sums, LY/src/odesolve/solver.rs:1965-2173):
 //   R_s = r + h c^_s v + h^2 sum_{j<s} (A.A)_sj A_j,   V_s = v + h sum_{j<s} a_sj A_j,
 // each component summed in ascending j.  The terms of a chunk of four
-// earlier stages are read together (one scalar load of the coefficients,
-// twelve LDS reads) and added without branches: a zero coefficient adds an
+// earlier stages are read together (eight coefficients from the LDS stage
+// rows, twelve stage-acceleration reads) and added without branches: a zero coefficient adds an
 // exact zero, so for finite stage accelerations the sums equal those that
 // skip zero coefficients bit for bit, and slots j >= s (stale) are replaced
 // by zero before use.  A non-finite stage acceleration makes the step's
@@ -100,12 +127,11 @@ struct MethodTraits<BQP_METHOD_RK87> {
 // of dependent scalar loads and branches that one term at a time costs.
 template <int METHOD>
 __device__ __forceinline__ void stage_sums(int s, double hs, const double* y, const double (*sA)[3][64], int lane,
-                                           double* ys) {
+                                           const double (*rows)[STAGE_ROW], double* ys) {
   constexpr int S = MethodTraits<METHOD>::S;
   constexpr int NCH = (S - 1 + STAGE_CHUNK - 1) / STAGE_CHUNK;
-  const SecondOrderTableau& SO = c_so[METHOD];
   const double hh = hs * hs;
-  const double hc = hs * SO.csum[s];
+  const double hc = hs * rows[s][0];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     ys[i] = fma(y[3 + i], hc, y[i]);
@@ -114,7 +140,7 @@ __device__ __forceinline__ void stage_sums(int s, double hs, const double* y, co
 #pragma unroll
   for (int k = 0; k < NCH; ++k) {
     if (STAGE_CHUNK * k < s) {
-      const double* cf = SO.sc[s][k];
+      const double* cf = rows[s] + 2 + 2 * STAGE_CHUNK * k;
       double A[STAGE_CHUNK][3];
 #pragma unroll
       for (int m = 0; m < STAGE_CHUNK; ++m) {
@@ -547,6 +573,9 @@ propagate_kernel(const PropParams p) {
   // its go (1) / stop (0) word
   __shared__ double sP[LEAD ? 3 : 1][64];
   __shared__ int sCmd;
+  __shared__ double sRows[S][STAGE_ROW];
+  stage_rows_to_lds<METHOD>(sRows, threadIdx.x, 64 * G);
+  __syncthreads();
   if constexpr (LEAD) {
     if (g != 0) {
       grav_worker<PREC, G>(p, g, lane, lds, sP, &sCmd);
@@ -755,10 +784,10 @@ propagate_kernel(const PropParams p) {
       const bool st = (s == 0) ? needk : stepping;
       if (s == 0 && wave_ballot(needk) == 0ull) continue;
       double ys[6];
-      stage_sums<METHOD>(s, hs, y, sA, lane, ys);
+      stage_sums<METHOD>(s, hs, y, sA, lane, sRows, ys);
       double as[3];
       if (SUR) sl.check_now = p.sur_check_every > 0 && (evals % p.sur_check_every) == 0;
-      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, st, bc, t + T.c[s] * hs, ys, as, sX, sO, sFlag, sl, el, sP);
+      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, st, bc, t + sRows[s][1] * hs, ys, as, sX, sO, sFlag, sl, el, sP);
       if (st) ++evals;
     }
     if (needk) have_k1 = true;

@@ -65,6 +65,7 @@ struct GroupLeader {
 
   const PropParams& p;
   double (*sA)[3][64];
+  const double (*rows)[STAGE_ROW];  // the method's stage rows in LDS (stage_rows_to_lds)
   int lane;
   bool compat, sampled;
   double q, inv_q;
@@ -81,8 +82,9 @@ struct GroupLeader {
   double hs = 0.0;
   double bst[3] = {0, 0, 0}, ysr[3] = {0, 0, 0};  // Encke: stage baseline position and deviation
 
-  __device__ __forceinline__ GroupLeader(const PropParams& p_, double (*sA_)[3][64], int lane_, bool alive_)
-      : p(p_), sA(sA_), lane(lane_), alive(alive_), eps(p_.eps) {
+  __device__ __forceinline__ GroupLeader(const PropParams& p_, double (*sA_)[3][64], const double (*rows_)[STAGE_ROW],
+                                         int lane_, bool alive_)
+      : p(p_), sA(sA_), rows(rows_), lane(lane_), alive(alive_), eps(p_.eps) {
     compat = (p.control == CONTROL_COMPAT);
     sampled = (p.t_eval != nullptr);
     q = compat ? 5.0 : (double)(c_tab[METHOD].order_err + 1);
@@ -269,12 +271,11 @@ struct GroupLeader {
   // Phase A, second half: the total (inertial) state of stage s_cur (second-
   // order form, as propagate_kernel) into yt; returns the stage time.
   __device__ __forceinline__ double stage_state(double* yt) {
-    const bqp_tableau& TB = c_tab[METHOD];
     const int s = s_cur;
     st = (s == 0) ? needk : stepping;
     double ys[6];
-    stage_sums<METHOD>(s, hs, y, sA, lane, ys);
-    const double tst = t + TB.c[s] * hs;
+    stage_sums<METHOD>(s, hs, y, sA, lane, rows, ys);
+    const double tst = t + rows[s][1] * hs;
     if constexpr (ENCKE) {
       double b6[6];
       two_body(p.mu, el.base, tst - el.t0, b6, el.x);
@@ -553,7 +554,10 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
   __shared__ int sAlive[TEAM_MAX];
   __shared__ int sNext;                       // next work item of the iteration
 
-  GroupLeader<METHOD, PREC> L(p, sA[leader ? team : 0], lane, leader);
+  __shared__ double sRows[S][STAGE_ROW];
+  stage_rows_to_lds<METHOD>(sRows, threadIdx.x, blockDim.x);
+  __syncthreads();
+  GroupLeader<METHOD, PREC> L(p, sA[leader ? team : 0], sRows, lane, leader);
   const unsigned first = leader ? team_first_group(blockIdx.x, (unsigned)team, gridDim.x) : 0u;
   const unsigned long long dealt_rows = 64ull * (unsigned long long)NT * gridDim.x;
   const bool drag_on = p.model.drag != DRAG_NONE;
@@ -713,7 +717,10 @@ propagate_pair_kernel(const PropParams p) {
   __shared__ double sPart[PAIR_GROUPS][PAIR_WAVES][3][64];  // gravity of each column share
   __shared__ int sAlive[PAIR_GROUPS];
 
-  GroupLeader<METHOD, PREC> L(p, sA[leader ? grp : 0], lane, leader);
+  __shared__ double sRows[S][STAGE_ROW];
+  stage_rows_to_lds<METHOD>(sRows, threadIdx.x, blockDim.x);
+  __syncthreads();
+  GroupLeader<METHOD, PREC> L(p, sA[leader ? grp : 0], sRows, lane, leader);
   const unsigned first = leader ? pair_first_group(blockIdx.x, (unsigned)grp, gridDim.x, (unsigned)p.deal_cus) : 0u;
   const unsigned long long dealt_rows = 64ull * PAIR_GROUPS * gridDim.x;
   // the leader's stage state and frame rotation, kept from phase A to C
