@@ -391,6 +391,34 @@ __device__ __forceinline__ void battin(double mu, const double* rb, const double
   az += k * fma(fq, rb[2], -dr[2]);
 }
 
+// Adds the partial sums of waves 1 .. G-1 (lds[q], written before a
+// barrier) to (gx, gy, gz) in wave order.  The reads of up to four partials
+// are issued before their adds, so the sum waits for one LDS round trip per
+// four partials instead of one per read.
+template <int G>
+__device__ __forceinline__ void add_partials(const double (*lds)[3][64], int lane, double& gx, double& gy,
+                                             double& gz) {
+  constexpr int B = 4;
+#pragma unroll
+  for (int q0 = 1; q0 < G; q0 += B) {
+    double P[B][3];
+#pragma unroll
+    for (int k = 0; k < B; ++k)
+      if (q0 + k < G) {
+        P[k][0] = lds[q0 + k][0][lane];
+        P[k][1] = lds[q0 + k][1][lane];
+        P[k][2] = lds[q0 + k][2][lane];
+      }
+#pragma unroll
+    for (int k = 0; k < B; ++k)
+      if (q0 + k < G) {
+        gx += P[k][0];
+        gy += P[k][1];
+        gz += P[k][2];
+      }
+  }
+}
+
 // Wave 0 alone runs the integrator when the group has more than one wave and
 // no network (PREC 0, 1, 4): the other waves only compute gravity columns.
 template <int PREC, int G>
@@ -492,14 +520,7 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
     gy = lds[0][1][lane];
     gz = lds[0][2][lane];
   }
-  if (G > 1) {
-#pragma unroll
-    for (int q = 1; q < G; ++q) {
-      gx += lds[q][0][lane];
-      gy += lds[q][1][lane];
-      gz += lds[q][2][lane];
-    }
-  }
+  if (G > 1) add_partials<G>(lds, lane, gx, gy, gz);
   // only wave 0 publishes the stage acceleration; the other waves' copies
   // would be identical, so they skip the inertial rotation and the drag
   if (!LEAD && G > 1 && g != 0) {
