@@ -1055,57 +1055,76 @@ constexpr int VALLADO_BANDS = 28;
 constexpr int kValladoH0Km[VALLADO_BANDS] = {0,   25,  30,  40,  50,  60,  70,  80,  90,  100,
                                              110, 120, 130, 140, 150, 180, 200, 250, 300, 350,
                                              400, 450, 500, 600, 700, 800, 900, 1000};
-static __constant__ double c_vallado_h0[VALLADO_BANDS] = {0,   25,  30,  40,  50,  60,  70,  80,  90,  100,
-                                                          110, 120, 130, 140, 150, 180, 200, 250, 300, 350,
-                                                          400, 450, 500, 600, 700, 800, 900, 1000};
-static __constant__ double c_vallado_rho0[28] = {
-    1.225,     3.899e-2,  1.774e-2,  3.972e-3,  1.057e-3,  3.206e-4,  8.770e-5,
-    1.905e-5,  3.396e-6,  5.297e-7,  9.661e-8,  2.438e-8,  8.484e-9,  3.845e-9,
-    2.070e-9,  5.464e-10, 2.789e-10, 7.248e-11, 2.418e-11, 9.518e-12, 3.725e-12,
-    1.585e-12, 6.967e-13, 1.454e-13, 3.614e-14, 1.170e-14, 5.245e-15, 3.019e-15};
-static __constant__ double c_vallado_H[28] = {7.249,  6.349,  6.682,  7.554,  8.382,  7.714,  6.549,
-                                       5.799,  5.382,  5.877,  7.263,  9.473,  12.636, 16.149,
-                                       22.523, 29.740, 37.105, 45.546, 53.628, 53.298, 58.515,
-                                       60.828, 63.822, 71.835, 88.667, 124.64, 181.05, 268.00};
 
 enum { DRAG_NONE = 0, DRAG_VALLADO = 1, DRAG_COMPAT = 2 };
 
+// The table as the kernels use it: per band base altitude (km), nominal
+// density (kg/m^3) and scale height (km), and the band of every 5-km step.
 // Every band base is a multiple of 5 km, so the band of an altitude in
-// [0, 1000) km is a function of floor(h / 5 km): entry k is the last band
-// whose base is at most 5k km.
-struct ValladoBand5 {
-  unsigned char band[200];
+// [0, 1000) km is a function of floor(h / 5 km): band5[k] is the last band
+// whose base is at most 5k km.  The kernels copy it to LDS at start
+// (vallado_to_lds): the band and its values are read per lane, and from
+// constant memory a lane-varying index would be two dependent vector-memory
+// round trips per evaluation.
+struct ValladoTable {
+  double h0[VALLADO_BANDS];
+  double rho0[VALLADO_BANDS];
+  double H[VALLADO_BANDS];
+  unsigned char band5[200];
 };
-constexpr ValladoBand5 make_vallado_band5() {
-  ValladoBand5 t{};
+constexpr ValladoTable make_vallado_table() {
+  ValladoTable t{{0,   25,  30,  40,  50,  60,  70,  80,  90,  100, 110, 120, 130, 140,
+                  150, 180, 200, 250, 300, 350, 400, 450, 500, 600, 700, 800, 900, 1000},
+                 {1.225,     3.899e-2,  1.774e-2,  3.972e-3,  1.057e-3,  3.206e-4,  8.770e-5,
+                  1.905e-5,  3.396e-6,  5.297e-7,  9.661e-8,  2.438e-8,  8.484e-9,  3.845e-9,
+                  2.070e-9,  5.464e-10, 2.789e-10, 7.248e-11, 2.418e-11, 9.518e-12, 3.725e-12,
+                  1.585e-12, 6.967e-13, 1.454e-13, 3.614e-14, 1.170e-14, 5.245e-15, 3.019e-15},
+                 {7.249,  6.349,  6.682,  7.554,  8.382,  7.714,  6.549,  5.799,  5.382,  5.877,
+                  7.263,  9.473,  12.636, 16.149, 22.523, 29.740, 37.105, 45.546, 53.628, 53.298,
+                  58.515, 60.828, 63.822, 71.835, 88.667, 124.64, 181.05, 268.00},
+                 {}};
   int i = 0;
   for (int k = 0; k < 200; ++k) {
     while (i + 1 < VALLADO_BANDS && kValladoH0Km[i + 1] <= 5 * k) ++i;
-    t.band[k] = (unsigned char)i;
+    t.band5[k] = (unsigned char)i;
   }
   return t;
 }
-static __constant__ ValladoBand5 c_vallado_band5 = make_vallado_band5();
+static constexpr ValladoTable k_vallado = make_vallado_table();
+
+// Copies the table to LDS; called by all threads of the workgroup (thread
+// tid of nthreads) before a barrier.
+__device__ __forceinline__ void vallado_to_lds(ValladoTable* t, int tid, int nthreads) {
+  for (int k = tid; k < 3 * VALLADO_BANDS; k += nthreads) {
+    const int b = k % VALLADO_BANDS;
+    if (k < VALLADO_BANDS)
+      t->h0[b] = k_vallado.h0[b];
+    else if (k < 2 * VALLADO_BANDS)
+      t->rho0[b] = k_vallado.rho0[b];
+    else
+      t->H[b] = k_vallado.H[b];
+  }
+  for (int k = tid; k < 200; k += nthreads) t->band5[k] = k_vallado.band5[k];
+}
 
 // Band of altitude hk (km): the last band whose base is at most hk (band 0
 // below 0 km and for NaN; the top band from 1000 km on).  floor(hk / 5) is
 // taken from a truncated product and corrected by exact comparisons with 5k
 // (multiples of 5 up to 1000 are exact doubles), so it is exact.
-__device__ __forceinline__ int vallado_band(double hk) {
+__device__ __forceinline__ int vallado_band(const ValladoTable& t, double hk) {
   if (hk >= 1000.0) return VALLADO_BANDS - 1;
   if (!(hk >= 0.0)) return 0;
   int k = (int)(hk * 0.2);
   if (k > 0 && 5.0 * k > hk) --k;
   if (k < 199 && 5.0 * (k + 1) <= hk) ++k;
-  return c_vallado_band5.band[k];
+  return t.band5[k];
 }
-
-__device__ __forceinline__ double density(int model, double h_m) {
+__device__ __forceinline__ double density(const ValladoTable& t, int model, double h_m) {
   const double hk = h_m * 1e-3;
   if (model == DRAG_VALLADO) {
-    if (hk < 0.0) return c_vallado_rho0[0];
-    const int b = vallado_band(hk);
-    return c_vallado_rho0[b] * exp(-(hk - c_vallado_h0[b]) / c_vallado_H[b]);
+    if (hk < 0.0) return t.rho0[0];
+    const int b = vallado_band(t, hk);
+    return t.rho0[b] * exp(-(hk - t.h0[b]) / t.H[b]);
   }
   if (model == DRAG_COMPAT) {
     if (hk <= 0.0) return 1.225;

@@ -465,7 +465,7 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
                                            double (*lds)[3][64], double (*sA)[3][64], int slot,
                                            bool store, double bc, double t, const double* y,
                                            double* acc, uint4* sX, float* sO, int* sFlag, SurLane& sl,
-                                           EnckeLane& el, double (*sP)[64]) {
+                                           EnckeLane& el, double (*sP)[64], const ValladoTable& vt) {
   constexpr bool SUR = (PREC == 2 || PREC == 3);
   constexpr bool ENCKE = (PREC == 4);
   constexpr bool LEAD = leader_mode<PREC, G>();
@@ -537,7 +537,7 @@ __device__ __forceinline__ void accel_eval(const PropParams& p, int g, int lane,
   double az = gz;
   if (p.model.drag != DRAG_NONE && bc != 0.0) {
     const double r = sqrt(fma(yt[0], yt[0], fma(yt[1], yt[1], yt[2] * yt[2])));
-    const double rho = density(p.model.drag, r - p.model.r_atm);
+    const double rho = density(vt, p.model.drag, r - p.model.r_atm);
     if (rho > 0.0) {
       const double vrx = fma(p.model.omega, yt[1], yt[3]);
       const double vry = fma(-p.model.omega, yt[0], yt[4]);
@@ -595,7 +595,9 @@ propagate_kernel(const PropParams p) {
   __shared__ double sP[LEAD ? 3 : 1][64];
   __shared__ int sCmd;
   __shared__ double sRows[S][STAGE_ROW];
+  __shared__ ValladoTable sVal;
   stage_rows_to_lds<METHOD>(sRows, threadIdx.x, 64 * G);
+  vallado_to_lds(&sVal, threadIdx.x, 64 * G);
   __syncthreads();
   if constexpr (LEAD) {
     if (g != 0) {
@@ -808,7 +810,7 @@ propagate_kernel(const PropParams p) {
       stage_sums<METHOD>(s, hs, y, sA, lane, sRows, ys);
       double as[3];
       if (SUR) sl.check_now = p.sur_check_every > 0 && (evals % p.sur_check_every) == 0;
-      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, st, bc, t + sRows[s][1] * hs, ys, as, sX, sO, sFlag, sl, el, sP);
+      accel_eval<PREC, G, S>(p, g, lane, lds, sA, s, st, bc, t + sRows[s][1] * hs, ys, as, sX, sO, sFlag, sl, el, sP, sVal);
       if (st) ++evals;
     }
     if (needk) have_k1 = true;

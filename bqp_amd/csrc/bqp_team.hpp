@@ -555,7 +555,9 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
   __shared__ int sNext;                       // next work item of the iteration
 
   __shared__ double sRows[S][STAGE_ROW];
+  __shared__ ValladoTable sVal;
   stage_rows_to_lds<METHOD>(sRows, threadIdx.x, blockDim.x);
+  vallado_to_lds(&sVal, threadIdx.x, blockDim.x);
   __syncthreads();
   GroupLeader<METHOD, PREC> L(p, sA[leader ? team : 0], sRows, lane, leader);
   const unsigned first = leader ? team_first_group(blockIdx.x, (unsigned)team, gridDim.x) : 0u;
@@ -641,7 +643,7 @@ __global__ void __launch_bounds__(64 * TEAM_WAVES) propagate_team_kernel(const P
           if (b != 0.0) {
             const double y0 = sYt[k][0][lane], y1 = sYt[k][1][lane], y2 = sYt[k][2][lane];
             const double r = sqrt(fma(y0, y0, fma(y1, y1, y2 * y2)));
-            const double rho = density(p.model.drag, r - p.model.r_atm);
+            const double rho = density(sVal, p.model.drag, r - p.model.r_atm);
             if (rho > 0.0) {
               const double vrx = fma(p.model.omega, y1, sYt[k][3][lane]);
               const double vry = fma(-p.model.omega, y0, sYt[k][4][lane]);
@@ -718,7 +720,9 @@ propagate_pair_kernel(const PropParams p) {
   __shared__ int sAlive[PAIR_GROUPS];
 
   __shared__ double sRows[S][STAGE_ROW];
+  __shared__ ValladoTable sVal;
   stage_rows_to_lds<METHOD>(sRows, threadIdx.x, blockDim.x);
+  vallado_to_lds(&sVal, threadIdx.x, blockDim.x);
   __syncthreads();
   GroupLeader<METHOD, PREC> L(p, sA[leader ? grp : 0], sRows, lane, leader);
   const unsigned first = leader ? pair_first_group(blockIdx.x, (unsigned)grp, gridDim.x, (unsigned)p.deal_cus) : 0u;
@@ -779,7 +783,7 @@ propagate_pair_kernel(const PropParams p) {
       double az = gz;
       if (p.model.drag != DRAG_NONE && L.bc != 0.0) {
         const double r = sqrt(fma(yt[0], yt[0], fma(yt[1], yt[1], yt[2] * yt[2])));
-        const double rho = density(p.model.drag, r - p.model.r_atm);
+        const double rho = density(sVal, p.model.drag, r - p.model.r_atm);
         if (rho > 0.0) {
           const double vrx = fma(p.model.omega, yt[1], yt[3]);
           const double vry = fma(-p.model.omega, yt[0], yt[4]);
