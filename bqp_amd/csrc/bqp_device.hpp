@@ -285,21 +285,30 @@ __device__ __forceinline__ void grav_fp64_pipelined(const FieldView& f, int g, d
         r += 4;
         kk -= 4;
       }
-      // the last 0..3 entries are issued before a0, a1 are computed
+      // the last 0..3 entries: two are loaded unconditionally while a0, a1
+      // are computed (as in the loop above, four entries live at once; more
+      // would spill scalar registers to VGPR lanes), a third afterwards.
+      // Past a column's end these read the next column's entries or the
+      // table's padding (at most two entries past it).
       BQP_WAIT_SMEM();
       BQP_SCHED_FENCE();
-      GravEntryL t0, t1, t2;
-      if (kk >= 1) t0 = r[0];
-      if (kk >= 2) t1 = r[1];
-      if (kk >= 3) t2 = r[2];
+      const GravEntryL t0 = r[0], t1 = r[1];
       BQP_SCHED_FENCE();
       BQP_COL_STEP(a0, v2, v1, w2, w1);
       BQP_COL_STEP(a1, v1, v2, w1, w2);
-      BQP_WAIT_SMEM();
-      BQP_SCHED_FENCE();
-      if (kk >= 1) BQP_COL_STEP(t0, v2, v1, w2, w1);
-      if (kk >= 2) BQP_COL_STEP(t1, v1, v2, w1, w2);
-      if (kk >= 3) BQP_COL_STEP(t2, v2, v1, w2, w1);
+      if (kk >= 1) {
+        BQP_WAIT_SMEM();
+        BQP_SCHED_FENCE();
+        const GravEntryL t2 = r[2];
+        BQP_SCHED_FENCE();
+        BQP_COL_STEP(t0, v2, v1, w2, w1);
+        if (kk >= 2) BQP_COL_STEP(t1, v1, v2, w1, w2);
+        if (kk >= 3) {
+          BQP_WAIT_SMEM();
+          BQP_SCHED_FENCE();
+          BQP_COL_STEP(t2, v2, v1, w2, w1);
+        }
+      }
     } else if (kk == 1) {
       BQP_COL_STEP(a0, v2, v1, w2, w1);
     }
