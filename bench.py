@@ -499,6 +499,8 @@ def run_population(args) -> dict:
     prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado",
                               r_min=bqp_amd.R_EARTH + bqp_amd.GROUND_ALTITUDE_M,
                               ground_flattening=bqp_amd.WGS84_FLATTENING)
+    if args.launch:
+        prop.set_launch(**json.loads(args.launch))
     for form in forms:
         if form.startswith("surrogate"):
             if sur is None:
