@@ -402,9 +402,14 @@ void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, const Launc
 // multiply) and the sectoral entry's six terms, plus 19 per recursion entry
 // (g rho, U and W, six sums).  The per-wave powers z^gap (at most 2G complex
 // multiplies per wave) are not counted.
-double grav_flops(int N, int jmax) {
+double grav_flops(int N, int jmax, bool factored = true) {
+  // set-up 16, column 0 9 per entry; per column j >= 1 18 (one complex
+  // multiply to its sectoral value, and either the fold of its six sums or
+  // the sectoral entry's six terms) plus per recursion entry 16 (factored
+  // sweep: one recursion, six sums) or 19 (split form: V and W recursions)
+  const double per_entry = factored ? 16.0 : 19.0;
   double f = 16.0 + 9.0 * (N + 1.0);
-  for (int j = 1; j <= jmax; ++j) f += 18.0 + 19.0 * (N + 1.0 - j);
+  for (int j = 1; j <= jmax; ++j) f += 18.0 + per_entry * (N + 1.0 - j);
   return f;
 }
 
@@ -1263,7 +1268,10 @@ int32_t bqp_surrogate_init(bqp_ctx* c, int32_t nlow, double r_lo, double r_hi, u
 // plus rotation 14 and, when enabled, drag 30.
 double bqp_flops_per_rhs(const bqp_ctx* c) {
   if (!c || c->nmax < 0) return 0.0;
-  double f = grav_flops(c->nmax, c->jmax);
+  // every form runs the factored sweep except the split form's (a, b) table
+  // (fields outside the degree range of its rescaled FP32 copy)
+  const bool factored = c->cfg.precision != BQP_PRECISION_SPLIT || c->d_tablf != nullptr;
+  double f = grav_flops(c->nmax, c->jmax, factored);
   f += 14.0;
   if (c->cfg.drag != BQP_DRAG_NONE) f += 30.0;
   return f;

@@ -19,6 +19,10 @@
 //   ax += XV*V + XW*W;  ay += YV*V + YW*W;  az += ZV*V + ZW*W
 //
 // For the column's first (sectoral) entry, a holds the sectoral factor.
+// The FP64 kernels use the rescaled form of this table (GravEntryL) and run
+// the recursion of columns j >= 1 once, on the factor Q(n) = V(n,j)/V(j,j)
+// = W(n,j)/W(j,j), multiplying each column's sums by its sectoral value at
+// the column's end (BQP_QSTEP).
 // All entries are wave-uniform, so the table is read through scalar loads
 // (SGPR operands), not per-lane memory traffic.
 #pragma once
@@ -143,6 +147,63 @@ static __constant__ float c_sectf[SECT_MAX];
     sz = fma((E).zv, VOUT, sz);          \
   }
 
+// Factored column sweep.  Within column j >= 1, V and W obey the same
+// recursion in n with the same coefficients and start from V(j-1,j) =
+// W(j-1,j) = 0, so V(n,j) = Q(n) V(j,j) and W(n,j) = Q(n) W(j,j), where
+// Q(j) = 1, Q(j-1) = 0 and Q(n) = zt Q(n-1) - g_n rho Q(n-2) (the rescaled
+// recursion above, run once instead of twice).  The column's six sums run
+// over Q alone and are multiplied by the sectoral value (V(j,j), W(j,j)) when
+// the column ends: 9 FP64 operations per entry instead of 11.
+#define BQP_QSTEP(E, QOUT, QPREV)        \
+  {                                      \
+    const double C_ = (E).g * rho;       \
+    QOUT = fma(zt, QPREV, -(C_ * QOUT)); \
+    sx = fma((E).xv, QOUT, sx);          \
+    tx = fma((E).xw, QOUT, tx);          \
+    sy = fma((E).yv, QOUT, sy);          \
+    ty = fma((E).yw, QOUT, ty);          \
+    sz = fma((E).zv, QOUT, sz);          \
+    tz = fma((E).zw, QOUT, tz);          \
+  }
+// Opens a column's sums with its sectoral entry (Q(j) = 1).
+#define BQP_QOPEN(E) \
+  {                  \
+    sx = (E).xv;     \
+    tx = (E).xw;     \
+    sy = (E).yv;     \
+    ty = (E).yw;     \
+    sz = (E).zv;     \
+    tz = (E).zw;     \
+  }
+// Opens a column's sums with its first recursion entry (Q(j+1) = zt).
+#define BQP_QFIRST(E)     \
+  {                       \
+    sx = (E).xv * zt;     \
+    tx = (E).xw * zt;     \
+    sy = (E).yv * zt;     \
+    ty = (E).yw * zt;     \
+    sz = (E).zv * zt;     \
+    tz = (E).zw * zt;     \
+  }
+// Adds the sectoral entry's terms (Q(j) = 1) to open sums.
+#define BQP_QSECT(E) \
+  {                  \
+    sx += (E).xv;    \
+    tx += (E).xw;    \
+    sy += (E).yv;    \
+    ty += (E).yw;    \
+    sz += (E).zv;    \
+    tz += (E).zw;    \
+  }
+// Adds a closed column, sums times its sectoral value (VS, WS), to the
+// running acceleration (AX, AY, AZ).
+#define BQP_QFOLD(VS, WS, AX, AY, AZ)  \
+  {                                    \
+    AX = fma(VS, sx, fma(WS, tx, AX)); \
+    AY = fma(VS, sy, fma(WS, ty, AY)); \
+    AZ = fma(VS, sz, fma(WS, tz, AZ)); \
+  }
+
 // (ar + i ai) *= (br + i bi)
 __device__ __forceinline__ void cmul(double& ar, double& ai, double br, double bi) {
   const double re = fma(ar, br, -(ai * bi));
@@ -253,6 +314,9 @@ __device__ __forceinline__ void grav_fp64_pipelined(const FieldView& f, int g, d
     ws = vs0 * zai;
     next_a = false;
   }
+  // column 0's terms (wave 0) open the running sums; the other columns are
+  // folded in as they close
+  double gx = sx, gy = sy, gz = sz;
   while (j <= f.jmax) {
     const CEntryL ec = e1 + (col_offset(f.nmax, j) - top);
     int kk = top - j;  // recursion entries of column j
@@ -263,8 +327,8 @@ __device__ __forceinline__ void grav_fp64_pipelined(const FieldView& f, int g, d
     BQP_SCHED_FENCE();
     BQP_WAIT_SMEM();
     BQP_SCHED_FENCE();
-    double v1 = vs, w1 = ws, v2 = 0.0, w2 = 0.0;
-    BQP_SECT_TERM(es, vs, ws);
+    double q1 = 1.0, q2 = 0.0;  // Q(j), Q(j-1)
+    BQP_QOPEN(es);
     CEntryL r = ec + 3;
     if (kk >= 2) {
       kk -= 2;  // entries after a0, a1
@@ -273,15 +337,15 @@ __device__ __forceinline__ void grav_fp64_pipelined(const FieldView& f, int g, d
         BQP_SCHED_FENCE();
         const GravEntryL b0 = r[0], b1 = r[1];
         BQP_SCHED_FENCE();
-        BQP_COL_STEP(a0, v2, v1, w2, w1);
-        BQP_COL_STEP(a1, v1, v2, w1, w2);
+        BQP_QSTEP(a0, q2, q1);
+        BQP_QSTEP(a1, q1, q2);
         BQP_WAIT_SMEM();
         BQP_SCHED_FENCE();
         a0 = r[2];
         a1 = r[3];
         BQP_SCHED_FENCE();
-        BQP_COL_STEP(b0, v2, v1, w2, w1);
-        BQP_COL_STEP(b1, v1, v2, w1, w2);
+        BQP_QSTEP(b0, q2, q1);
+        BQP_QSTEP(b1, q1, q2);
         r += 4;
         kk -= 4;
       }
@@ -294,32 +358,33 @@ __device__ __forceinline__ void grav_fp64_pipelined(const FieldView& f, int g, d
       BQP_SCHED_FENCE();
       const GravEntryL t0 = r[0], t1 = r[1];
       BQP_SCHED_FENCE();
-      BQP_COL_STEP(a0, v2, v1, w2, w1);
-      BQP_COL_STEP(a1, v1, v2, w1, w2);
+      BQP_QSTEP(a0, q2, q1);
+      BQP_QSTEP(a1, q1, q2);
       if (kk >= 1) {
         BQP_WAIT_SMEM();
         BQP_SCHED_FENCE();
         const GravEntryL t2 = r[2];
         BQP_SCHED_FENCE();
-        BQP_COL_STEP(t0, v2, v1, w2, w1);
-        if (kk >= 2) BQP_COL_STEP(t1, v1, v2, w1, w2);
+        BQP_QSTEP(t0, q2, q1);
+        if (kk >= 2) BQP_QSTEP(t1, q1, q2);
         if (kk >= 3) {
           BQP_WAIT_SMEM();
           BQP_SCHED_FENCE();
-          BQP_COL_STEP(t2, v2, v1, w2, w1);
+          BQP_QSTEP(t2, q2, q1);
         }
       }
     } else if (kk == 1) {
-      BQP_COL_STEP(a0, v2, v1, w2, w1);
+      BQP_QSTEP(a0, q2, q1);
     }
+    BQP_QFOLD(vs, ws, gx, gy, gz);
     const double pr = next_a ? zar : zbr, pi = next_a ? zai : zbi;
     j += next_a ? ga : gb;
     next_a = !next_a;
     cmul(vs, ws, pr, pi);
   }
-  ax = sx + tx;
-  ay = sy + ty;
-  az = sz + tz;
+  ax = gx;
+  ay = gy;
+  az = gz;
 }
 
 // FP64 gravity, body-fixed position (m) -> acceleration (m/s^2), for the
@@ -389,62 +454,62 @@ __device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, d
     ws = vs0 * zai;
     next_a = false;
   }
+  // column 0's terms (wave 0) open the running sums; the other columns are
+  // folded in as they close
+  double gx = sx, gy = sy, gz = sz;
   while (j <= f.jmax) {
     const int len = top + 1 - j;
     const CEntryL ec = e1 + (col_offset(f.nmax, j) - top);
     // The sectoral entry shares its scalar-load batch with the column's first
     // three recursion entries (one wait for four entries; its terms are added
     // after the three steps, so the compiler cannot hoist its load and wait
-    // above the branch); the three steps run with the registers swapped so
-    // the latest value ends in v1.
+    // above the branch).  The first recursion entry has Q(j+1) = zt, so its
+    // terms are plain products that open the sums.
     CEntryL r = ec + 1;
     int kk = len - 1;
-    double v1, w1, v2, w2;
+    double q1, q2;
     if (kk >= 3) {
       const GravEntryL es = ec[0], e0 = r[0], e1q = r[1], e2 = r[2];
-      v2 = vs;
-      w2 = ws;
-      v1 = 0.0;
-      w1 = 0.0;
-      BQP_COL_STEP(e0, v1, v2, w1, w2);
-      BQP_COL_STEP(e1q, v2, v1, w2, w1);
-      BQP_COL_STEP(e2, v1, v2, w1, w2);
-      BQP_SECT_TERM(es, vs, ws);
+      q1 = zt;   // Q(j+1)
+      q2 = 1.0;  // Q(j)
+      BQP_QFIRST(e0);
+      BQP_QSTEP(e1q, q2, q1);
+      BQP_QSTEP(e2, q1, q2);
+      BQP_QSECT(es);
       kk -= 3;
       r += 3;
     } else {
       const GravEntryL es = ec[0];
-      BQP_SECT_TERM(es, vs, ws);
-      v1 = vs;
-      w1 = ws;
-      v2 = 0.0;
-      w2 = 0.0;
+      BQP_QOPEN(es);
+      q1 = 1.0;
+      q2 = 0.0;
     }
     for (; kk >= 4; kk -= 4, r += 4) {
       const GravEntryL e0 = r[0], e1q = r[1], e2 = r[2], e3 = r[3];
-      BQP_COL_STEP(e0, v2, v1, w2, w1);
-      BQP_COL_STEP(e1q, v1, v2, w1, w2);
-      BQP_COL_STEP(e2, v2, v1, w2, w1);
-      BQP_COL_STEP(e3, v1, v2, w1, w2);
+      BQP_QSTEP(e0, q2, q1);
+      BQP_QSTEP(e1q, q1, q2);
+      BQP_QSTEP(e2, q2, q1);
+      BQP_QSTEP(e3, q1, q2);
     }
     for (; kk >= 2; kk -= 2, r += 2) {
       const GravEntryL e0 = r[0], e1q = r[1];
-      BQP_COL_STEP(e0, v2, v1, w2, w1);
-      BQP_COL_STEP(e1q, v1, v2, w1, w2);
+      BQP_QSTEP(e0, q2, q1);
+      BQP_QSTEP(e1q, q1, q2);
     }
     if (kk) {
       const GravEntryL e0 = r[0];
-      BQP_COL_STEP(e0, v2, v1, w2, w1);
+      BQP_QSTEP(e0, q2, q1);
     }
+    BQP_QFOLD(vs, ws, gx, gy, gz);
     // next owned column (wave-uniform gap; the select is on a scalar flag)
     const double pr = next_a ? zar : zbr, pi = next_a ? zai : zbi;
     j += next_a ? ga : gb;
     next_a = !next_a;
     cmul(vs, ws, pr, pi);
   }
-  ax = sx + tx;
-  ay = sy + ty;
-  az = sz + tz;
+  ax = gx;
+  ay = gy;
+  az = gz;
 }
 
 // FP32 recursion step of the split form ((a, b) entries); the new value
@@ -640,20 +705,35 @@ constexpr int SPLIT_RESCALED_MAX_N = 90;
     qy = fmaf((E).yv, VOUT, qy);           \
     qz = fmaf((E).zv, VOUT, qz);           \
   }
-// One step of either form (column 0 has no W half).
+// FP32 step of the factored sweep (as BQP_QSTEP): one recursion Q, the
+// column's V-coefficient sums in q, W-coefficient sums in u.
+#define BQP_QSTEP_LF(E, QOUT, QPREV)           \
+  {                                            \
+    const float C_ = (E).g * rhof;             \
+    QOUT = fmaf(ztf, QPREV, -(C_ * QOUT));     \
+    qx = fmaf((E).xv, QOUT, qx);               \
+    ux = fmaf((E).xw, QOUT, ux);               \
+    qy = fmaf((E).yv, QOUT, qy);               \
+    uy = fmaf((E).yw, QOUT, uy);               \
+    qz = fmaf((E).zv, QOUT, qz);               \
+    uz = fmaf((E).zw, QOUT, uz);               \
+  }
+// One step of either form: column 0 (no W half) carries V itself, the other
+// columns carry the factored Q (W arguments unused).
 #define BQP_STEP_LF(E, VOUT, VPREV, WOUT, WPREV) \
   {                                              \
     if constexpr (COL0) {                        \
       BQP_COL0_STEP_LF(E, VOUT, VPREV);          \
     } else {                                     \
-      BQP_COL_STEP_LF(E, VOUT, VPREV, WOUT, WPREV); \
+      BQP_QSTEP_LF(E, VOUT, VPREV);              \
     }                                            \
   }
 
 // FP32 recursion entries r[0..cnt) of one column, four per batch with the
 // next batch in flight.  a0..a3 hold r[0..3] on entry (loaded by the caller
 // with the column's head, or past the column's end when cnt < 4: the table
-// is padded).  (v1, w1) is the latest value, (v2, w2) the one before.
+// is padded).  v1 is the latest value (V in column 0, Q in the others), v2
+// the one before; w1, w2 are not used.
 template <bool COL0>
 __device__ __forceinline__ void col_tail_lf(CEntryLF r, int cnt, GravEntryLF a0, GravEntryLF a1, GravEntryLF a2,
                                             GravEntryLF a3, float ztf, float rhof, float& v1, float& v2, float& w1,
@@ -703,14 +783,15 @@ __device__ __forceinline__ void col_tail_lf(CEntryLF r, int cnt, GravEntryLF a0,
 }
 
 // Split-precision gravity over the rescaled tables (degree PIPE_MIN_N ..
-// SPLIT_RESCALED_MAX_N): the same column sweep, sectoral jumps and term
-// grouping as grav_fp64_pipelined.  Entries with n <= SPLIT_N (the central
+// SPLIT_RESCALED_MAX_N): the same column sweep, sectoral jumps and factored
+// columns as grav_fp64_pipelined.  Entries with n <= SPLIT_N (the central
 // term and J2..J4) run in FP64 from the FP64 table; the recursion continues
-// in FP32 from the FP32 copy (U values converted where a column crosses
-// SPLIT_N; sectoral values reached in FP64 and converted for columns past
-// SPLIT_N).  Each column's FP32 terms accumulate in two FP32 chains per
-// component, whose sum is folded into an FP32 Neumaier-compensated
-// accumulator; value and compensation are added to the FP64 sums at the end.
+// in FP32 from the FP32 copy (Q values converted where a column crosses
+// SPLIT_N).  Each column's FP32 terms accumulate in six FP32 sums over Q;
+// the column's result, those sums times its sectoral value, is formed and
+// accumulated across columns in FP64 (rounding 2^-53 per column, below the
+// Neumaier-compensated FP32 accumulator this replaced, and no FP32 range
+// limit on the sectoral values).
 template <int G>
 __device__ __forceinline__ void grav_split_pipelined(const FieldView& f, int g, double x, double y, double z,
                                                      double& ax, double& ay, double& az) {
@@ -721,7 +802,7 @@ __device__ __forceinline__ void grav_split_pipelined(const FieldView& f, int g, 
   const double vs0 = R / sqrt(r2);
   const float ztf = (float)zt, rhof = (float)rho;
   double sx = 0.0, sy = 0.0, sz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
-  Comp32 cx = {0.f, 0.f}, cy = {0.f, 0.f}, cz = {0.f, 0.f};
+  double gx = 0.0, gy = 0.0, gz = 0.0;  // FP64 accumulation of the column results
   const int top = f.nmax + 1;
   CEntryL e = (CEntryL)f.tabl;
   CEntryLF ef = (CEntryLF)f.tablf;
@@ -744,11 +825,13 @@ __device__ __forceinline__ void grav_split_pipelined(const FieldView& f, int g, 
     const GravEntryLF a0 = r[0], a1 = r[1], a2 = r[2], a3 = r[3];
     BQP_SCHED_FENCE();
     col_tail_lf<true>(r, top - SPLIT_N, a0, a1, a2, a3, ztf, rhof, fv1, fv2, fw1, fw2, qx, ux, qy, uy, qz, uz);
-    comp_add(cx, qx);
-    comp_add(cy, qy);
-    comp_add(cz, qz);
+    gx = sx + (double)qx;
+    gy = sy + (double)qy;
+    gz = sz + (double)qz;
   }
-  // columns j >= 1 by sectoral jumps (FP64), as in grav_fp64
+  // columns j >= 1 by sectoral jumps (FP64), as in grav_fp64, each as the
+  // factored sweep: Q(n) with Q(j) = 1, the FP64 head (n <= SPLIT_N) and the
+  // FP32 rest summed per column, then multiplied by the sectoral value in FP64
   const int ga = 2 * G - 1 - 2 * g, gb = 2 * g + 1;
   double zgr = 1.0, zgi = 0.0;  // z^g
   for (int k = 0; k < g; ++k) cmul(zgr, zgi, xt, yt);
@@ -769,7 +852,7 @@ __device__ __forceinline__ void grav_split_pipelined(const FieldView& f, int g, 
   while (j <= f.jmax) {
     const int off = col_offset(f.nmax, j);  // the column's sectoral entry
     const int kk = top - j;                 // its recursion entries
-    float fv1, fv2, fw1, fw2;
+    float fv1, fv2, fw1 = 0.f, fw2 = 0.f;
     float qx = 0.f, ux = 0.f, qy = 0.f, uy = 0.f, qz = 0.f, uz = 0.f;
     if (j <= SPLIT_N) {
       // FP64 sectoral term and recursion entries n = j+1 .. SPLIT_N
@@ -779,29 +862,28 @@ __device__ __forceinline__ void grav_split_pipelined(const FieldView& f, int g, 
       BQP_SCHED_FENCE();
       const GravEntryL es = ec[0];
       BQP_SCHED_FENCE();
-      BQP_SECT_TERM(es, vs, ws);
-      double v1 = vs, w1 = ws, v2 = 0.0, w2 = 0.0;
+      BQP_QOPEN(es);
+      double q1 = 1.0, q2 = 0.0;
       for (int i = 0; i < h; ++i) {
         const GravEntryL E = ec[1 + i];
-        BQP_COL_STEP(E, v2, v1, w2, w1);
-        const double tv = v1, tw = w1;  // latest back into v1 / w1
-        v1 = v2;
-        w1 = w2;
-        v2 = tv;
-        w2 = tw;
+        BQP_QSTEP(E, q2, q1);
+        const double tq = q1;  // latest back into q1
+        q1 = q2;
+        q2 = tq;
       }
-      fv1 = (float)v1;
-      fw1 = (float)w1;
-      fv2 = (float)v2;
-      fw2 = (float)w2;
+      fv1 = (float)q1;
+      fv2 = (float)q2;
       CEntryLF r = ef + off + 1 + h;
       BQP_WAIT_SMEM();
       BQP_SCHED_FENCE();
       const GravEntryLF a0 = r[0], a1 = r[1], a2 = r[2], a3 = r[3];
       BQP_SCHED_FENCE();
       col_tail_lf<false>(r, kk - h, a0, a1, a2, a3, ztf, rhof, fv1, fv2, fw1, fw2, qx, ux, qy, uy, qz, uz);
+      gx = fma(vs, sx + (double)qx, fma(ws, tx + (double)ux, gx));
+      gy = fma(vs, sy + (double)qy, fma(ws, ty + (double)uy, gy));
+      gz = fma(vs, sz + (double)qz, fma(ws, tz + (double)uz, gz));
     } else {
-      // all FP32: the sectoral entry is loaded with the first four recursion entries
+      // all FP32: the sectoral entry (Q = 1) is loaded with the first four recursion entries
       CEntryLF ec = ef + off;
       BQP_WAIT_SMEM();
       BQP_SCHED_FENCE();
@@ -810,30 +892,27 @@ __device__ __forceinline__ void grav_split_pipelined(const FieldView& f, int g, 
       BQP_SCHED_FENCE();
       BQP_WAIT_SMEM();
       BQP_SCHED_FENCE();
-      const float vsf = (float)vs, wsf = (float)ws;
-      qx = es.xv * vsf;
-      ux = es.xw * wsf;
-      qy = es.yv * vsf;
-      uy = es.yw * wsf;
-      qz = es.zv * vsf;
-      uz = es.zw * wsf;
-      fv1 = vsf;
-      fw1 = wsf;
+      qx = es.xv;
+      ux = es.xw;
+      qy = es.yv;
+      uy = es.yw;
+      qz = es.zv;
+      uz = es.zw;
+      fv1 = 1.f;
       fv2 = 0.f;
-      fw2 = 0.f;
       col_tail_lf<false>(ec + 1, kk, a0, a1, a2, a3, ztf, rhof, fv1, fv2, fw1, fw2, qx, ux, qy, uy, qz, uz);
+      gx = fma(vs, (double)qx, fma(ws, (double)ux, gx));
+      gy = fma(vs, (double)qy, fma(ws, (double)uy, gy));
+      gz = fma(vs, (double)qz, fma(ws, (double)uz, gz));
     }
-    comp_add(cx, qx + ux);
-    comp_add(cy, qy + uy);
-    comp_add(cz, qz + uz);
     const double pr = next_a ? zar : zbr, pi = next_a ? zai : zbi;
     j += next_a ? ga : gb;
     next_a = !next_a;
     cmul(vs, ws, pr, pi);
   }
-  ax = (sx + tx) + ((double)cx.s + (double)cx.c);
-  ay = (sy + ty) + ((double)cy.s + (double)cy.c);
-  az = (sz + tz) + ((double)cz.s + (double)cz.c);
+  ax = gx;
+  ay = gy;
+  az = gz;
 }
 
 // Split-precision gravity: the rescaled pipelined form where its FP32 table
@@ -879,9 +958,10 @@ __device__ __forceinline__ void cpow_bits(double zr, double zi, int k, double& r
   }
 }
 
-// Terms of column j >= 1 with sectoral value (vs, ws): the pipelined entry
-// loop of grav_fp64_pipelined (reads at most two entries past the column).
-#define BQP_COL_PIPE(EC, KK, VS, WS)                        \
+// Terms of column j >= 1 with sectoral value (vs, ws), folded into (AX, AY,
+// AZ): the factored pipelined entry loop of grav_fp64_pipelined (reads at
+// most two entries past the column).
+#define BQP_COL_PIPE(EC, KK, VS, WS, AX, AY, AZ)            \
   {                                                         \
     int kk = (KK);                                          \
     BQP_WAIT_SMEM();                                        \
@@ -891,8 +971,8 @@ __device__ __forceinline__ void cpow_bits(double zr, double zi, int k, double& r
     BQP_SCHED_FENCE();                                      \
     BQP_WAIT_SMEM();                                        \
     BQP_SCHED_FENCE();                                      \
-    double v1 = (VS), w1 = (WS), v2 = 0.0, w2 = 0.0;        \
-    BQP_SECT_TERM(es, (VS), (WS));                          \
+    double q1 = 1.0, q2 = 0.0;                              \
+    BQP_QOPEN(es);                                          \
     CEntryL r = (EC) + 3;                                   \
     if (kk >= 2) {                                          \
       kk -= 2;                                              \
@@ -901,15 +981,15 @@ __device__ __forceinline__ void cpow_bits(double zr, double zi, int k, double& r
         BQP_SCHED_FENCE();                                  \
         const GravEntryL b0 = r[0], b1 = r[1];              \
         BQP_SCHED_FENCE();                                  \
-        BQP_COL_STEP(a0, v2, v1, w2, w1);                   \
-        BQP_COL_STEP(a1, v1, v2, w1, w2);                   \
+        BQP_QSTEP(a0, q2, q1);                              \
+        BQP_QSTEP(a1, q1, q2);                              \
         BQP_WAIT_SMEM();                                    \
         BQP_SCHED_FENCE();                                  \
         a0 = r[2];                                          \
         a1 = r[3];                                          \
         BQP_SCHED_FENCE();                                  \
-        BQP_COL_STEP(b0, v2, v1, w2, w1);                   \
-        BQP_COL_STEP(b1, v1, v2, w1, w2);                   \
+        BQP_QSTEP(b0, q2, q1);                              \
+        BQP_QSTEP(b1, q1, q2);                              \
         r += 4;                                             \
         kk -= 4;                                            \
       }                                                     \
@@ -920,16 +1000,17 @@ __device__ __forceinline__ void cpow_bits(double zr, double zi, int k, double& r
       if (kk >= 2) t1 = r[1];                               \
       if (kk >= 3) t2 = r[2];                               \
       BQP_SCHED_FENCE();                                    \
-      BQP_COL_STEP(a0, v2, v1, w2, w1);                     \
-      BQP_COL_STEP(a1, v1, v2, w1, w2);                     \
+      BQP_QSTEP(a0, q2, q1);                                \
+      BQP_QSTEP(a1, q1, q2);                                \
       BQP_WAIT_SMEM();                                      \
       BQP_SCHED_FENCE();                                    \
-      if (kk >= 1) BQP_COL_STEP(t0, v2, v1, w2, w1);        \
-      if (kk >= 2) BQP_COL_STEP(t1, v1, v2, w1, w2);        \
-      if (kk >= 3) BQP_COL_STEP(t2, v2, v1, w2, w1);        \
+      if (kk >= 1) BQP_QSTEP(t0, q2, q1);                   \
+      if (kk >= 2) BQP_QSTEP(t1, q1, q2);                   \
+      if (kk >= 3) BQP_QSTEP(t2, q2, q1);                   \
     } else if (kk == 1) {                                   \
-      BQP_COL_STEP(a0, v2, v1, w2, w1);                     \
+      BQP_QSTEP(a0, q2, q1);                                \
     }                                                       \
+    BQP_QFOLD((VS), (WS), AX, AY, AZ);                      \
   }
 
 // Column 0 (zonal, W = 0) in 4-entry batches with the next batch in flight
@@ -991,7 +1072,8 @@ __device__ __forceinline__ void grav_col0_pipe(CEntryL e, int top, double vs0, d
 __device__ __forceinline__ void grav_unit(const FieldView& f, int u, double xt, double yt, double zt, double rho,
                                           double vs0, double z32r, double z32i, double& px, double& py,
                                           double& pz) {
-  double sx = 0.0, sy = 0.0, sz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
+  double sx = 0.0, sy = 0.0, sz = 0.0, tx, ty, tz;  // column sums
+  double gx = 0.0, gy = 0.0, gz = 0.0;              // the unit's running sum
   const int top = f.nmax + 1;
   CEntryL e = (CEntryL)f.tabl;
   CEntryL e1 = e + top;  // column 1
@@ -999,7 +1081,7 @@ __device__ __forceinline__ void grav_unit(const FieldView& f, int u, double xt, 
     int j = path == 0 ? u : 2 * GRAV_UNITS - 1 - u;
     double vs, ws;
     if (j == 0) {  // path A of unit 0 opens with the zonal column
-      grav_col0_pipe(e, top, vs0, zt, rho, sx, sy, sz);
+      grav_col0_pipe(e, top, vs0, zt, rho, gx, gy, gz);
       j = 2 * GRAV_UNITS;
       vs = vs0 * z32r;
       ws = vs0 * z32i;
@@ -1011,13 +1093,13 @@ __device__ __forceinline__ void grav_unit(const FieldView& f, int u, double xt, 
     }
     for (; j <= f.jmax; j += 2 * GRAV_UNITS) {
       const CEntryL ec = e1 + (col_offset(f.nmax, j) - top);
-      BQP_COL_PIPE(ec, top - j, vs, ws);
+      BQP_COL_PIPE(ec, top - j, vs, ws, gx, gy, gz);
       cmul(vs, ws, z32r, z32i);
     }
   }
-  px = sx + tx;
-  py = sy + ty;
-  pz = sz + tz;
+  px = gx;
+  py = gy;
+  pz = gz;
 }
 
 // A position prepared for unit sums: the inertial position turned into the
