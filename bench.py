@@ -668,6 +668,8 @@ def run_form(form, args, field, bufs, n, warm_passes):
         "flops_per_rhs": frhs,
         "achieved_tflops": achieved,
         "launch": getattr(prop, "launch", None),
+        # the persistent kernel and shape the timed passes ran with
+        "launch_used": prop.last_launch(),
         # per-row work spread: every lane holds one row, so the rows with
         # the most evaluations set the launch's length
         "evals_per_row": {"min": int(ev_rows.min()), "mean": float(ev_rows.mean()), "max": int(ev_rows.max())},
@@ -804,7 +806,7 @@ def main():
             "unit": "TFLOP/s",
             "frac": base["achieved_tflops"] / FP64_PEAK_TFLOPS,
             "traffic": None,
-            "kernel": "bqp::propagate_kernel<RK87, FP64, G>",
+            "kernel": kernel_name(base.get("launch_used"), head),
             "kernel_ms": base["kernel_ms"],
             "flops_per_rhs": base["flops_per_rhs"],
             "rhs_evals": base["rhs_evals"],
@@ -876,6 +878,18 @@ def copy_calibration(mib: int):
     rc = hip().hipMemcpy(C.c_void_p(b.ptr), C.c_void_p(a.ptr), a.nbytes, D2D)
     synchronize()
     print(f"[bench] calibration copy of {mib} MiB ({a.nbytes} bytes), rc {rc}", file=sys.stderr, flush=True)
+
+
+def kernel_name(used, form: str) -> str:
+    """Name of the persistent kernel a call ran, as the kernel trace shows it."""
+    prec = {"fp64": "0", "split": "1", "encke": "4"}.get(form, "0")
+    if not used:
+        return "bqp::propagate_kernel<2, 0, G>"
+    if used["kernel"] == "pair":
+        return f"bqp::propagate_pair_kernel<2, {prec}> (two 64-trajectory groups per 8-wave workgroup)"
+    if used["kernel"] == "team":
+        return f"bqp::propagate_team_kernel<2, {prec}> ({used['teams']} groups per 16-wave workgroup)"
+    return f"bqp::propagate_kernel<2, {prec}, {used['waves_per_group']}>"
 
 
 def committed_traffic():

@@ -98,6 +98,19 @@ class Launch(C.Structure):
     ]
 
 
+class LaunchInfo(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_int32),
+        ("waves_per_group", C.c_int32),
+        ("teams", C.c_int32),
+        ("blocks", C.c_int32),
+        ("threads", C.c_int32),
+    ]
+
+
+# include/bqp.h BQP_KERNEL_*
+KERNELS = {0: "none", 1: "fixed", 2: "team", 3: "pair"}
+
 # every symbol include/bqp.h declares (checked by the ABI tests)
 EXPORTS = [
     "bqp_default_config",
@@ -116,6 +129,7 @@ EXPORTS = [
     "bqp_get_tableau",
     "bqp_flops_per_rhs",
     "bqp_get_cost_model",
+    "bqp_last_launch",
     "bqp_surrogate_param_count",
     "bqp_surrogate_init",
     "bqp_surrogate_get_params",
@@ -171,6 +185,7 @@ def load(path: str | None = None):
     L.bqp_flops_per_rhs.restype = C.c_double
     L.bqp_flops_per_rhs.argtypes = [C.c_void_p]
     L.bqp_get_cost_model.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.bqp_last_launch.argtypes = [C.c_void_p, C.POINTER(LaunchInfo)]
     _fp = C.POINTER(C.c_float)
     vp = C.c_void_p
     L.bqp_surrogate_param_count.restype = C.c_int64

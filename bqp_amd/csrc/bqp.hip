@@ -426,6 +426,30 @@ int grid_for(bqp_ctx* c, prop_fn kern, int G, int64_t n, int* blocks) {
   return BQP_OK;
 }
 
+// Records a launch shape as the context's last one (bqp_last_launch).
+void note_launch(bqp_ctx* c, const LaunchShape& ls) {
+  bqp_launch_info& li = c->last_launch;
+  li.kernel = ls.teams == 0 ? BQP_KERNEL_FIXED : (ls.G == PAIR_WAVES ? BQP_KERNEL_PAIR : BQP_KERNEL_TEAM);
+  li.waves_per_group = ls.G;
+  li.teams = ls.teams;
+  li.blocks = ls.blocks;
+  li.threads = ls.threads;
+}
+
+// Whether an automatic launch (waves_per_group = teams_per_block = 0) takes
+// the pair kernel: fields of degree >= PAIR_AUTO_MIN_N whose batch is one
+// round of G = 4 groups (each lane holds one row, so the launch ends with
+// groups running alone on their CUs, which the pair kernel runs on eight
+// waves instead of four).  Measured on 65,536 LEO rows x 1 d with drag, pair
+// against G = 4: degree 40 -18%, 50 -3%, 60 +5%, 70 +12% (7 d: +12.4%); at
+// degree 70 with 32,768 rows +14%, with 98,304 rows (lanes refill) -2%;
+// Encke at degree 70 +7%.  The pair kernel's results equal G = 8 bit for bit.
+bool auto_pair(const bqp_ctx* c, const bqp_launch& L, int64_t n) {
+  if (L.waves_per_group != 0 || L.teams_per_block != 0 || c->nmax < PAIR_AUTO_MIN_N) return false;
+  const int64_t groups = (n + 63) / 64;
+  return default_g(c, n) == 4 && groups <= (int64_t)c->cus * 4;
+}
+
 // The team kernel (bqp_team.hpp) takes FP64 Cowell and Encke calls that ask
 // for it (teams_per_block 1, 2 or 4 with an automatic group width) on fields
 // of degree >= PIPE_MIN_N: one 16-wave workgroup per CU carrying that many
@@ -436,7 +460,9 @@ int shape_for(bqp_ctx* c, int kprec, int64_t n, LaunchShape* out) {
   const bqp_launch& L = c->launch;
   const int64_t groups = (n + 63) / 64;
   *out = LaunchShape{};
-  if (L.waves_per_group == PAIR_WAVES && L.teams_per_block == PAIR_GROUPS && (kprec == 0 || kprec == 4)) {
+  const bool pair_ok = (kprec == 0 || kprec == 4);
+  const bool pair_asked = L.waves_per_group == PAIR_WAVES && L.teams_per_block == PAIR_GROUPS;
+  if ((pair_asked && pair_ok) || (pair_ok && auto_pair(c, L, n))) {
     out->kern = (kprec == 4) ? pick_pair_encke(c->cfg.method) : pick_pair_fp64(c->cfg.method);
     if (!out->kern) return fail(c, BQP_E_INVALID, "unknown method");
     const int64_t need = (groups + PAIR_GROUPS - 1) / PAIR_GROUPS;
@@ -526,6 +552,7 @@ int run_surrogate_partitioned(bqp_ctx* c, PropParams p, int64_t n, int kprec, bq
   LaunchShape se, ss;
   if (cnt[0] > 0 && (rc = shape_for(c, 0, cnt[0], &se))) return rc;
   if (cnt[1] > 0 && (rc = shape_for(c, kprec, cnt[1], &ss))) return rc;
+  note_launch(c, cnt[0] > 0 ? se : ss);
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, p.sur_totals, 5);
   HIP_TRY(c, hipEventRecord(c->ev0, c->stream));
   if (cnt[0] > 0) {
@@ -679,6 +706,7 @@ int run_propagation(bqp_ctx* c, PropParams p, int64_t n, bqp_stats* stats, const
     const int rc = shape_for(c, kprec, n, &ls);
     if (rc) return rc;
   }
+  note_launch(c, ls);
 
   if (c->launch.use_graph && !fixed) {
     GraphKey key;
@@ -896,6 +924,12 @@ int32_t bqp_set_launch(bqp_ctx* c, const bqp_launch* l) {
 }
 
 const char* bqp_last_error(const bqp_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int32_t bqp_last_launch(const bqp_ctx* c, bqp_launch_info* out) {
+  if (!c || !out) return BQP_E_INVALID;
+  *out = c->last_launch;
+  return BQP_OK;
+}
 
 int32_t bqp_get_cost_model(bqp_ctx* c, int64_t* evals, double* revolutions, int64_t* rows) {
   if (!c || !c->d_cost) return BQP_E_INVALID;

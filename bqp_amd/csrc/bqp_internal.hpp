@@ -73,6 +73,7 @@ struct bqp_ctx {
   GraphKey* graph_key = nullptr;
   int64_t graph_builds = 0;
   bqp_mlp_state* mlp = nullptr;
+  bqp_launch_info last_launch{};  // shape of the last propagation call
   std::string err;
 };
 

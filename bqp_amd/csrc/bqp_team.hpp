@@ -54,6 +54,8 @@ constexpr int TEAM_SLOTS = 2 * GRAV_UNITS;
 // Pair kernel: waves per workgroup = column shares per group, and groups.
 constexpr int PAIR_WAVES = 8;
 constexpr int PAIR_GROUPS = 2;
+// Lowest field degree at which an automatic launch takes the pair kernel.
+constexpr int PAIR_AUTO_MIN_N = 60;
 
 // The integrator state of one group's 64 rows, held by the group's leader
 // wave, and its per-evaluation steps.  sA is the group's LDS stage storage.

@@ -161,6 +161,15 @@ class Propagator:
         self.launch = {"waves_per_group": waves_per_group, "use_graph": use_graph, "blocks": blocks,
                        "sort_by_cost": sort_by_cost, "teams_per_block": teams_per_block}
 
+    def last_launch(self) -> dict:
+        """Shape of the last propagation call: the persistent kernel that ran
+        ("fixed", "team" or "pair"; "none" before any call), its group width,
+        groups per workgroup, grid and workgroup size."""
+        info = nat.LaunchInfo()
+        self._check(self._lib.bqp_last_launch(self._ctx, C.byref(info)))
+        return {"kernel": nat.KERNELS.get(info.kernel, str(info.kernel)), "waves_per_group": info.waves_per_group,
+                "teams": info.teams, "blocks": info.blocks, "threads": info.threads}
+
     def cost_model(self) -> dict:
         """The learned cost model (sort_by_cost = 1): per (altitude bin,
         cos i bin) cell, the evaluations, revolutions and rows summed over

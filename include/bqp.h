@@ -118,7 +118,10 @@ typedef struct bqp_stats {
 /* Launch shape (tuning); zero fields mean "default".  Chosen by the
  * autotuner in bqp_amd.autotune when not given. */
 typedef struct bqp_launch {
-  int32_t waves_per_group;  /* G: waves sharing one 64-trajectory group (0 = auto, 1, 2, 4, 8, 16) */
+  int32_t waves_per_group;  /* G: waves sharing one 64-trajectory group (0 = auto, 1, 2, 4, 8, 16).
+                               Auto (with teams_per_block 0) picks G from the batch size; FP64 and
+                               Encke calls on fields of degree >= 60 whose batch is one round of
+                               G = 4 groups take the pair kernel instead (results equal G = 8) */
   int32_t use_graph;        /* 1 = capture the call's launch sequence in a HIP graph once and
                                replay it while buffers, sizes and settings are unchanged */
   int32_t blocks;           /* persistent grid size; 0 = fill the device */
@@ -223,6 +226,23 @@ double bqp_flops_per_rhs(const bqp_ctx* ctx);
  * configuration was last set.  Each array holds
  * BQP_COST_ALT_BINS * BQP_COST_INC_BINS values; any may be NULL. */
 int32_t bqp_get_cost_model(bqp_ctx* ctx, int64_t* evals, double* revolutions, int64_t* rows);
+
+/* Launch shape of the context's last propagation call (no counterpart in the
+ * reference, which has no launch shape): which persistent kernel ran and
+ * how.  A checked-surrogate call reports its exact-row launch when it had
+ * one, else its surrogate launch. */
+#define BQP_KERNEL_NONE 0   /* no propagation call yet */
+#define BQP_KERNEL_FIXED 1  /* fixed-G kernel: waves_per_group waves per 64-trajectory group */
+#define BQP_KERNEL_TEAM 2   /* team kernel: teams groups per 16-wave workgroup */
+#define BQP_KERNEL_PAIR 3   /* pair kernel: 2 groups per 8-wave workgroup, results equal G = 8 */
+typedef struct bqp_launch_info {
+  int32_t kernel;           /* BQP_KERNEL_* */
+  int32_t waves_per_group;  /* G of the fixed-G kernel; 8 for the pair kernel; 0 for the team kernel */
+  int32_t teams;            /* groups per workgroup (team and pair kernels), else 0 */
+  int32_t blocks;           /* persistent grid */
+  int32_t threads;          /* workgroup size */
+} bqp_launch_info;
+int32_t bqp_last_launch(const bqp_ctx* ctx, bqp_launch_info* out);
 
 /* ------------------------------------------------------------------------
  * Learned gravity surrogate.  No counterpart in the reference (which always

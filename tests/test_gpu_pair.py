@@ -118,3 +118,43 @@ def test_launch_combinations_are_checked():
                     {"waves_per_group": 16, "teams_per_block": 1}):
             with pytest.raises(Exception, match="teams_per_block"):
                 p.set_launch(**bad)
+
+
+def test_automatic_launch_takes_the_pair_kernel_where_it_measured_faster():
+    """The automatic launch (waves_per_group = teams_per_block = 0) takes the
+    pair kernel for FP64 and Encke fields of degree >= 60 whose batch is one
+    round of G = 4 groups (65,536 rows at most on 256 CUs), the fixed-G
+    kernel otherwise; the automatic pair call equals an explicit one bit for
+    bit."""
+    n = 65536
+    w = inputs.workload("w1", n=n, seed=43)
+    tf = np.full(n, 120.0)
+    f70 = bqp_amd.kaula(70)
+    announce("automatic launch: degree 70 / 20, 65,536 and 98,304 rows, FP64, Encke, split")
+    with bqp_amd.Propagator(f70, method="rk87", rtol=1e-12, drag="vallado") as p:
+        ya, sa, _ = p.propagate_batch_final(w["y0"], 0.0, tf, w["bc"])
+        used = p.last_launch()
+        assert used["kernel"] == "pair" and used["waves_per_group"] == 8 and used["teams"] == 2
+        assert used["threads"] == 512 and used["blocks"] == n // 128
+        p.set_launch(**PAIR)
+        yb, sb, _ = p.propagate_batch_final(w["y0"], 0.0, tf, w["bc"])
+        assert np.all(sa == 0) and np.array_equal(sa, sb) and np.array_equal(ya, yb)
+        p.set_launch(waves_per_group=4)
+        p.propagate_batch_final(w["y0"], 0.0, tf, w["bc"])
+        assert p.last_launch()["kernel"] == "fixed" and p.last_launch()["waves_per_group"] == 4
+    with bqp_amd.Propagator(f70, method="rk87", control="absolute", eps=1e-6, formulation="encke") as p:
+        p.propagate_batch_final(w["y0"][:4096], 0.0, tf[:4096])
+        assert p.last_launch()["kernel"] == "fixed"  # 64 groups: G = 16
+        p.propagate_batch_final(w["y0"], 0.0, tf)
+        assert p.last_launch()["kernel"] == "pair"
+    with bqp_amd.Propagator(f70, method="rk87", rtol=1e-12, precision="split") as p:
+        p.propagate_batch_final(w["y0"], 0.0, tf)
+        used = p.last_launch()
+        assert used["kernel"] == "fixed" and used["waves_per_group"] == 4 and used["threads"] == 256
+    with bqp_amd.Propagator(bqp_amd.kaula(20), method="rk87", rtol=1e-12) as p:
+        p.propagate_batch_final(w["y0"], 0.0, tf)
+        assert p.last_launch()["kernel"] == "fixed"
+    w2 = inputs.workload("w1", n=98304, seed=44)
+    with bqp_amd.Propagator(f70, method="rk87", rtol=1e-12) as p:
+        p.propagate_batch_final(w2["y0"], 0.0, np.full(98304, 60.0))
+        assert p.last_launch()["kernel"] == "fixed" and p.last_launch()["waves_per_group"] == 4

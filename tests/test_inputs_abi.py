@@ -198,6 +198,9 @@ def test_host_only_entry_points_without_gpu():
         _native.E_INVALID
     assert L.bqp_last_error(None) == b"null context"
     assert L.bqp_get_cost_model(None, None, None, None) == _native.E_INVALID
+    assert L.bqp_last_launch(None, None) == _native.E_INVALID
+    info = _native.LaunchInfo()
+    assert L.bqp_last_launch(None, ctypes.byref(info)) == _native.E_INVALID
 
 
 def test_sampled_call_returns_only_the_samples_a_failed_arc_reached():
