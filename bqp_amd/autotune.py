@@ -3,7 +3,7 @@
 Tunables (``bqp_launch``): G, the number of waves that share one group of 64
 trajectories (1, 2, 4, 8, 16; the workgroup is 64·G threads and holds 64
 trajectories), the persistent grid size, the longest-first hand-out order,
-and -- for FP64 / Encke -- the pair kernel (two groups per 8-wave
+and -- for FP64, split and Encke -- the pair kernel (two groups per 8-wave
 workgroup, ``waves_per_group=8, teams_per_block=2``) and, for fields of
 degree >= 30, the team kernel with 1, 2 or 4 groups per 16-wave workgroup
 (``teams_per_block``), whose waves move to the groups still running.  The tuner runs a short pilot of the real batch — every row's span
@@ -55,7 +55,7 @@ def autotune(prop, y0, t0, tf, bc=None, *, pilot_fraction: float = 0.01, min_pil
             return {"signature": key, "cached": True, **hit}
     if candidates is None:
         candidates = [{"waves_per_group": g, "sort_by_cost": s} for g in (1, 2, 4, 8, 16) for s in (1, 0)]
-        if prop.config.precision == 0:
+        if prop.config.precision in (0, 1):
             candidates.append({"waves_per_group": 8, "teams_per_block": 2, "sort_by_cost": 1})
         if prop.field.nmax >= 30 and prop.config.precision == 0:
             candidates += [{"waves_per_group": 0, "teams_per_block": t, "sort_by_cost": 1} for t in (1, 2, 4)]

@@ -443,7 +443,8 @@ void note_launch(bqp_ctx* c, const LaunchShape& ls) {
 // waves instead of four).  Measured on 65,536 LEO rows x 1 d with drag, pair
 // against G = 4: degree 40 -18%, 50 -3%, 60 +5%, 70 +12% (7 d: +12.4%); at
 // degree 70 with 32,768 rows +14%, with 98,304 rows (lanes refill) -2%;
-// Encke at degree 70 +7%.  The pair kernel's results equal G = 8 bit for bit.
+// Encke at degree 70 +7%; split form at degree 70 +10% (1 d), +11% (7 d),
+// degree 50 +4%.  The pair kernel's results equal G = 8 bit for bit.
 bool auto_pair(const bqp_ctx* c, const bqp_launch& L, int64_t n) {
   if (L.waves_per_group != 0 || L.teams_per_block != 0 || c->nmax < PAIR_AUTO_MIN_N) return false;
   const int64_t groups = (n + 63) / 64;
@@ -453,17 +454,19 @@ bool auto_pair(const bqp_ctx* c, const bqp_launch& L, int64_t n) {
 // The team kernel (bqp_team.hpp) takes FP64 Cowell and Encke calls that ask
 // for it (teams_per_block 1, 2 or 4 with an automatic group width) on fields
 // of degree >= PIPE_MIN_N: one 16-wave workgroup per CU carrying that many
-// groups.  The pair kernel takes FP64 Cowell and Encke calls with
+// groups.  The pair kernel takes FP64 and split Cowell and Encke calls with
 // waves_per_group 8 and teams_per_block 2: two groups per 8-wave workgroup.
 // Other calls take the fixed-G kernel.
 int shape_for(bqp_ctx* c, int kprec, int64_t n, LaunchShape* out) {
   const bqp_launch& L = c->launch;
   const int64_t groups = (n + 63) / 64;
   *out = LaunchShape{};
-  const bool pair_ok = (kprec == 0 || kprec == 4);
+  const bool pair_ok = (kprec == 0 || kprec == 1 || kprec == 4);
   const bool pair_asked = L.waves_per_group == PAIR_WAVES && L.teams_per_block == PAIR_GROUPS;
   if ((pair_asked && pair_ok) || (pair_ok && auto_pair(c, L, n))) {
-    out->kern = (kprec == 4) ? pick_pair_encke(c->cfg.method) : pick_pair_fp64(c->cfg.method);
+    out->kern = (kprec == 4)   ? pick_pair_encke(c->cfg.method)
+                : (kprec == 1) ? pick_pair_split(c->cfg.method)
+                               : pick_pair_fp64(c->cfg.method);
     if (!out->kern) return fail(c, BQP_E_INVALID, "unknown method");
     const int64_t need = (groups + PAIR_GROUPS - 1) / PAIR_GROUPS;
     int64_t b = L.blocks;

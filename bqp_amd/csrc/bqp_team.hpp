@@ -710,7 +710,7 @@ propagate_pair_kernel(const PropParams p) {
   using MT = MethodTraits<METHOD>;
   constexpr int S = MT::S;
   constexpr bool ENCKE = (PREC == 4);
-  static_assert(PREC == 0 || PREC == 4, "pair kernel: plain FP64 Cowell or Encke");
+  static_assert(PREC == 0 || PREC == 1 || PREC == 4, "pair kernel: FP64 or split Cowell, or Encke");
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const bool leader = w < PAIR_GROUPS;  // wave g leads group g
@@ -762,7 +762,10 @@ propagate_pair_kernel(const PropParams p) {
       for (int k = 0; k < nsh; ++k) {
         const int share = sh0 + k;
         double gx, gy, gz;
-        grav_fp64<PAIR_WAVES>(p.field, share, xb, yb, zb, gx, gy, gz);
+        if constexpr (PREC == 1)
+          grav_split<PAIR_WAVES>(p.field, share, xb, yb, zb, gx, gy, gz);
+        else
+          grav_fp64<PAIR_WAVES>(p.field, share, xb, yb, zb, gx, gy, gz);
         sPart[g][share][0][lane] = gx;
         sPart[g][share][1][lane] = gy;
         sPart[g][share][2][lane] = gz;

@@ -23,10 +23,12 @@ prop_fn pick_sur(int method, int prec, int G);  // prec 2 (BF16 network) or 3 (F
 prop_fn pick_sur_bf16(int method, int G);
 prop_fn pick_sur_fp8(int method, int G);
 prop_fn pick_encke(int method, int G);
-// team and pair kernels (bqp_team.hpp) for a method, FP64 Cowell or Encke
+// team kernels (bqp_team.hpp) for a method, FP64 Cowell or Encke; pair
+// kernels also for the split form
 prop_fn pick_team_fp64(int method);
 prop_fn pick_team_encke(int method);
 prop_fn pick_pair_fp64(int method);
+prop_fn pick_pair_split(int method);
 prop_fn pick_pair_encke(int method);
 
 hipError_t upload_fp64(const double* sect, const float* sectf);

@@ -13,4 +13,13 @@ prop_fn pick_split(int method, int G) {
     default: return nullptr;
   }
 }
+
+prop_fn pick_pair_split(int method) {
+  switch (method) {
+    case BQP_METHOD_DOPRI5: return propagate_pair_kernel<BQP_METHOD_DOPRI5, 1>;
+    case BQP_METHOD_DOP853: return propagate_pair_kernel<BQP_METHOD_DOP853, 1>;
+    case BQP_METHOD_RK87: return propagate_pair_kernel<BQP_METHOD_RK87, 1>;
+    default: return nullptr;
+  }
+}
 }  // namespace bqp

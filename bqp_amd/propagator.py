@@ -154,8 +154,10 @@ class Propagator:
         fixed-G kernel; 1, 2 or 4 with G = 0 = the team kernel, whose
         results do not depend on the launch shape or the batch, FP64 or
         Encke, fields of degree >= 30; 2 with G = 8 = the pair kernel, two
-        groups per 8-wave workgroup whose results equal G = 8's, FP64 or
-        Encke)."""
+        groups per 8-wave workgroup whose results equal G = 8's, FP64, split
+        or Encke).  The automatic launch (G = 0, groups 0) takes the pair
+        kernel for fields of degree >= 60 whose batch is one round of
+        4-wave groups; `last_launch` reports what a call ran."""
         L = nat.Launch(waves_per_group, use_graph, blocks, sort_by_cost, teams_per_block)
         self._check(self._lib.bqp_set_launch(self._ctx, C.byref(L)))
         self.launch = {"waves_per_group": waves_per_group, "use_graph": use_graph, "blocks": blocks,

@@ -119,9 +119,10 @@ typedef struct bqp_stats {
  * autotuner in bqp_amd.autotune when not given. */
 typedef struct bqp_launch {
   int32_t waves_per_group;  /* G: waves sharing one 64-trajectory group (0 = auto, 1, 2, 4, 8, 16).
-                               Auto (with teams_per_block 0) picks G from the batch size; FP64 and
-                               Encke calls on fields of degree >= 60 whose batch is one round of
-                               G = 4 groups take the pair kernel instead (results equal G = 8) */
+                               Auto (with teams_per_block 0) picks G from the batch size; FP64,
+                               split and Encke calls on fields of degree >= 60 whose batch is one
+                               round of G = 4 groups take the pair kernel instead (results equal
+                               G = 8) */
   int32_t use_graph;        /* 1 = capture the call's launch sequence in a HIP graph once and
                                replay it while buffers, sizes and settings are unchanged */
   int32_t blocks;           /* persistent grid size; 0 = fill the device */
@@ -137,7 +138,7 @@ typedef struct bqp_launch {
                                gravity work of whichever groups are still running; its results
                                do not depend on grid, group width or batch composition (bit for
                                bit), but it measures slower on the 65,536-row headline batch.
-                               2 with waves_per_group = 8 (FP64 or Encke): the pair kernel, two
+                               2 with waves_per_group = 8 (FP64, split or Encke): the pair kernel, two
                                groups per 8-wave workgroup, a group's gravity in the 8 column
                                shares of G = 8 so that it runs on all 8 waves once its partner is
                                done; results equal waves_per_group = 8 (bit for bit) */

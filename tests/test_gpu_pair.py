@@ -1,14 +1,16 @@
 """GPU tests of the pair kernel (bqp_team.hpp): two 64-trajectory groups per
 8-wave workgroup, each group's gravity in the eight column shares of the
 fixed-G kernel at G = 8, so that a group whose partner is done runs on all
-eight waves.  Taken by FP64 and Encke calls with waves_per_group = 8 and
-teams_per_block = 2.
+eight waves.  Taken by FP64, split and Encke calls with waves_per_group = 8
+and teams_per_block = 2, and by the automatic launch where it measured
+faster.
 
 * Every result -- final states, step, evaluation and reject counts, status
   -- equals the fixed-G kernel's at G = 8 bit for bit: RK8(7)13M with
   Vallado drag on ragged durations with more rows than the dealt round
   (queue refills), DOPRI5 (FSAL reuse), DOP853, the compat controller,
-  Encke, sampled output, failing rows, and a batch smaller than one pair.
+  split precision, Encke, sampled output, failing rows, and a batch smaller
+  than one pair.
 * against the oracle to the usual bars.
 """
 import numpy as np
@@ -77,6 +79,21 @@ def test_pair_encke_and_sampled_equal_fixed_g8():
     assert np.array_equal(res[0][1], res[1][1])
 
 
+@pytest.mark.parametrize("nmax", [40, 20])
+def test_pair_split_equals_fixed_g8(nmax):
+    """Split precision: degree 40 takes the rescaled FP32 table, degree 20
+    the (a, b) table; both equal the fixed kernel at G = 8."""
+    n = 70_000
+    w = inputs.workload("w1", n=n, seed=45)
+    rng = np.random.default_rng(45)
+    tf = rng.uniform(600.0, 2 * 3600.0, n)
+    announce(f"pair kernel vs fixed G=8: split precision, N={nmax}, drag, {n} rows, ragged 10 min-2 h")
+    a, b = _both(bqp_amd.kaula(nmax), w["y0"], 0.0, tf, w["bc"], method="rk87", rtol=1e-11, drag="vallado",
+                 precision="split")
+    assert np.all(a[1] == 0)
+    _assert_same(a, b)
+
+
 def test_pair_failing_rows_and_tiny_batches():
     w = inputs.workload("w1", n=200, seed=43)
     field = bqp_amd.kaula(20)
@@ -122,10 +139,10 @@ def test_launch_combinations_are_checked():
 
 def test_automatic_launch_takes_the_pair_kernel_where_it_measured_faster():
     """The automatic launch (waves_per_group = teams_per_block = 0) takes the
-    pair kernel for FP64 and Encke fields of degree >= 60 whose batch is one
-    round of G = 4 groups (65,536 rows at most on 256 CUs), the fixed-G
-    kernel otherwise; the automatic pair call equals an explicit one bit for
-    bit."""
+    pair kernel for FP64, split and Encke fields of degree >= 60 whose batch
+    is one round of G = 4 groups (65,536 rows at most on 256 CUs), the
+    fixed-G kernel otherwise; the automatic pair call equals an explicit one
+    (FP64) and the fixed kernel at G = 8 (split) bit for bit."""
     n = 65536
     w = inputs.workload("w1", n=n, seed=43)
     tf = np.full(n, 120.0)
@@ -148,9 +165,12 @@ def test_automatic_launch_takes_the_pair_kernel_where_it_measured_faster():
         p.propagate_batch_final(w["y0"], 0.0, tf)
         assert p.last_launch()["kernel"] == "pair"
     with bqp_amd.Propagator(f70, method="rk87", rtol=1e-12, precision="split") as p:
-        p.propagate_batch_final(w["y0"], 0.0, tf)
-        used = p.last_launch()
-        assert used["kernel"] == "fixed" and used["waves_per_group"] == 4 and used["threads"] == 256
+        ya, sa, _ = p.propagate_batch_final(w["y0"], 0.0, tf)
+        assert p.last_launch()["kernel"] == "pair"
+        p.set_launch(waves_per_group=8)
+        yb, sb, _ = p.propagate_batch_final(w["y0"], 0.0, tf)
+        assert p.last_launch()["kernel"] == "fixed" and p.last_launch()["waves_per_group"] == 8
+        assert np.all(sa == 0) and np.array_equal(sa, sb) and np.array_equal(ya, yb)
     with bqp_amd.Propagator(bqp_amd.kaula(20), method="rk87", rtol=1e-12) as p:
         p.propagate_batch_final(w["y0"], 0.0, tf)
         assert p.last_launch()["kernel"] == "fixed"
