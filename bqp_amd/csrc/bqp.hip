@@ -396,17 +396,15 @@ void enqueue_propagation(bqp_ctx* c, const PropParams& p, int64_t n, const Launc
   enqueue_cost_learn(c, p, n);
 }
 
-// Algorithmic FLOPs of one FP64 gravity evaluation in the rescaled recursion
+// Algorithmic FLOPs of one gravity evaluation in the rescaled recursion
 // (FMA = 2, other operations 1): set-up 16, column 0 9 per entry (g rho, U,
 // three sums), column j >= 1 18 for its sectoral value (one complex
-// multiply) and the sectoral entry's six terms, plus 19 per recursion entry
-// (g rho, U and W, six sums).  The per-wave powers z^gap (at most 2G complex
-// multiplies per wave) are not counted.
+// multiply) and either the fold of its six sums times the sectoral value
+// (factored sweep) or the sectoral entry's six terms, plus per recursion
+// entry 16 (factored sweep: g rho, Q, six sums) or 19 (the split form's
+// (a, b) table: g rho, V and W, six sums).  The per-wave powers z^gap (at
+// most 2G complex multiplies per wave) are not counted.
 double grav_flops(int N, int jmax, bool factored = true) {
-  // set-up 16, column 0 9 per entry; per column j >= 1 18 (one complex
-  // multiply to its sectoral value, and either the fold of its six sums or
-  // the sectoral entry's six terms) plus per recursion entry 16 (factored
-  // sweep: one recursion, six sums) or 19 (split form: V and W recursions)
   const double per_entry = factored ? 16.0 : 19.0;
   double f = 16.0 + 9.0 * (N + 1.0);
   for (int j = 1; j <= jmax; ++j) f += 18.0 + per_entry * (N + 1.0 - j);
