@@ -45,10 +45,10 @@ def test_w0_dopri5_vs_oracle(orc):
     assert np.mean(info["steps_per_row"] == steps) > 0.9
 
 
-def _oracle_ulp_spread(orc, field, w, n, method, rtol, drag, tf, all_components=True):
+def _oracle_ulp_spread(orc, field, w, n, method, rtol, drag, tf, all_components=True, components=(0,)):
     """The oracle against itself with one component of the initial state moved
-    by one ulp, each of the six components up and down (12 runs; only x0 up
-    with all_components=False); per row the largest displacement: the
+    by one ulp, each of the six components up and down (12 runs; with
+    all_components=False only the listed components, up); per row the largest displacement: the
     conditioning floor any two correct implementations can be expected to
     meet.  One perturbation alone underestimates it on sensitive rows (a row
     whose x0 + 1 ulp moves it by 3 mm moves by 3 cm under vx + 1 ulp)."""
@@ -57,7 +57,7 @@ def _oracle_ulp_spread(orc, field, w, n, method, rtol, drag, tf, all_components=
     y0 = w["y0"][:n]
     a, _, _, _ = orc.propagate_batch(model, integ, y0, 0.0, tf, bc)
     spread = np.zeros(n)
-    for comp in range(6 if all_components else 1):
+    for comp in (range(6) if all_components else components):
         for d in ((np.inf, -np.inf) if all_components else (np.inf,)):
             yp = y0.copy()
             yp[:, comp] = np.nextafter(yp[:, comp], d)
@@ -118,16 +118,32 @@ def test_w1_seven_day_n70_vs_oracle(orc):
 
 
 def test_w1_seven_day_n70_drag_vs_oracle_spread(orc):
-    announce("w1 N=70 drag 7 days, 8 rows vs oracle and its 1-ulp spread")
-    w = inputs.workload("w1", n=8)
+    """The headline problem with drag: 64 rows strided across the 65,536-row
+    workload (every altitude), 7 days, degree 70, Vallado drag, in the pair
+    kernel (the automatic launch of the full batch; its numerics are G = 8's).
+    The piecewise scale heights make the final state sensitive to ulp-level
+    differences, so the yardstick is the oracle against itself with x0 or vy
+    moved by one ulp: the worst row within 3x the worst spread plus 1 cm, and no
+    more rows beyond 1 cm than twice the oracle's own count plus two."""
+    announce("w1 N=70 drag 7 days, 64 strided rows (pair kernel) vs oracle and its 1-ulp spread")
+    w = inputs.workload("w1", n=65536)
+    idx = np.arange(0, 65536, 1024)
+    n = idx.size
+    ws = {"y0": w["y0"][idx], "bc": w["bc"][idx], "tf": np.full(n, 7 * DAY)}
     field = bqp_amd.kaula(70)
-    yg, sg, info, yo, so, steps = _run_both(orc, field, w, 8, "rk87", 1e-12, "vallado")
+    yg, sg, info, yo, so, steps = _run_both(orc, field, ws, n, "rk87", 1e-12, "vallado",
+                                            launch={"waves_per_group": 8, "teams_per_block": 2})
     assert np.all(sg == 0) and np.all(so == 0)
     err = pos_err(yg, yo)
-    spread = _oracle_ulp_spread(orc, field, w, 8, "rk87", 1e-12, "vallado", 7 * DAY, all_components=False)
-    announce(f"w1 drag parity max {err.max():.3e} median {np.median(err):.3e}; "
-             f"oracle 1-ulp spread max {spread.max():.3e}")
+    # x0 and vy (along-track for most rows) moved by one ulp, separately
+    spread = _oracle_ulp_spread(orc, field, ws, n, "rk87", 1e-12, "vallado", 7 * DAY, all_components=False,
+                                components=(0, 4))
+    beyond, beyond_o = int(np.sum(err > 1e-2)), int(np.sum(spread > 1e-2))
+    announce(f"w1 drag parity max {err.max():.3e} median {np.median(err):.3e}, {beyond}/{n} rows beyond 1 cm; "
+             f"oracle 1-ulp spread (x0, vy) max {spread.max():.3e} median {np.median(spread):.3e}, "
+             f"{beyond_o}/{n} beyond 1 cm")
     assert err.max() <= 3.0 * spread.max() + 1e-2
+    assert beyond <= 2 * beyond_o + 2
 
 
 @pytest.mark.parametrize("G", [1, 2, 4, 8, 16])
