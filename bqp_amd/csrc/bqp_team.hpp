@@ -704,6 +704,34 @@ __device__ __forceinline__ unsigned pair_first_group(unsigned b, unsigned k, uns
   return k == 0u ? r0 : 2u * nb - 1u - r0;
 }
 
+// Phase B of the pair kernel for wave w: with two live groups, shares
+// 2(w/2) and 2(w/2)+1 of group w % 2; with one, share w of it.  Each share's
+// partial gravity goes to sPart.  Returns false when no group is live.
+template <int PREC>
+__device__ __forceinline__ bool pair_shares(const PropParams& p, int w, int lane, const double (*sP)[3][64],
+                                            double (*sPart)[PAIR_WAVES][3][64], const int* sAlive) {
+  const int a0 = __builtin_amdgcn_readfirstlane(sAlive[0]);
+  const int a1 = __builtin_amdgcn_readfirstlane(sAlive[1]);
+  if (a0 + a1 == 0) return false;
+  const bool both = (a0 + a1) == 2;
+  const int g = both ? (w & 1) : (a0 ? 0 : 1);
+  const int sh0 = both ? 2 * (w >> 1) : w;
+  const int nsh = both ? 2 : 1;
+  const double xb = sP[g][0][lane], yb = sP[g][1][lane], zb = sP[g][2][lane];
+  for (int k = 0; k < nsh; ++k) {
+    const int share = sh0 + k;
+    double gx, gy, gz;
+    if constexpr (PREC == 1)
+      grav_split<PAIR_WAVES>(p.field, share, xb, yb, zb, gx, gy, gz);
+    else
+      grav_fp64<PAIR_WAVES>(p.field, share, xb, yb, zb, gx, gy, gz);
+    sPart[g][share][0][lane] = gx;
+    sPart[g][share][1][lane] = gy;
+    sPart[g][share][2][lane] = gz;
+  }
+  return true;
+}
+
 template <int METHOD, int PREC>
 __global__ void __launch_bounds__(64 * PAIR_WAVES) __attribute__((amdgpu_waves_per_eu(BQP_WAVES_PER_EU)))
 propagate_pair_kernel(const PropParams p) {
@@ -726,8 +754,19 @@ propagate_pair_kernel(const PropParams p) {
   stage_rows_to_lds<METHOD>(sRows, threadIdx.x, blockDim.x);
   vallado_to_lds(&sVal, threadIdx.x, blockDim.x);
   __syncthreads();
-  GroupLeader<METHOD, PREC> L(p, sA[leader ? grp : 0], sRows, lane, leader);
-  const unsigned first = leader ? pair_first_group(blockIdx.x, (unsigned)grp, gridDim.x, (unsigned)p.deal_cus) : 0u;
+  if (!leader) {
+    // waves 2..7 only compute column shares: the leaders' two barriers per
+    // round, with no integrator state live across the column sweeps (so
+    // the sweeps keep their scalar registers; in the shared loop the
+    // compiler restored 16 spilled SGPRs from VGPR lanes after every column)
+    while (true) {
+      __syncthreads();  // A: positions and live flags published
+      if (!pair_shares<PREC>(p, w, lane, sP, sPart, sAlive)) return;
+      __syncthreads();  // B: partial sums written
+    }
+  }
+  GroupLeader<METHOD, PREC> L(p, sA[grp], sRows, lane, true);
+  const unsigned first = pair_first_group(blockIdx.x, (unsigned)grp, gridDim.x, (unsigned)p.deal_cus);
   const unsigned long long dealt_rows = 64ull * PAIR_GROUPS * gridDim.x;
   // the leader's stage state and frame rotation, kept from phase A to C
   double yt[6] = {0, 0, 0, 0, 0, 0};
@@ -735,47 +774,23 @@ propagate_pair_kernel(const PropParams p) {
 
   while (true) {
     // ------------------------------------------------------------ A
-    if (leader) {
-      L.begin(first, dealt_rows);
-      if (L.alive) {
-        const double tst = L.stage_state(yt);
-        sincos(fma(p.model.omega, tst, p.model.theta0), &sth, &cth);
-        sP[grp][0][lane] = fma(cth, yt[0], sth * yt[1]);
-        sP[grp][1][lane] = fma(cth, yt[1], -(sth * yt[0]));
-        sP[grp][2][lane] = yt[2];
-      }
-      if (lane == 0) sAlive[grp] = L.alive ? 1 : 0;
+    L.begin(first, dealt_rows);
+    if (L.alive) {
+      const double tst = L.stage_state(yt);
+      sincos(fma(p.model.omega, tst, p.model.theta0), &sth, &cth);
+      sP[grp][0][lane] = fma(cth, yt[0], sth * yt[1]);
+      sP[grp][1][lane] = fma(cth, yt[1], -(sth * yt[0]));
+      sP[grp][2][lane] = yt[2];
     }
-    __syncthreads();
+    if (lane == 0) sAlive[grp] = L.alive ? 1 : 0;
+    __syncthreads();  // A
     // ------------------------------------------------------------ B
-    // two live groups: wave w computes shares 2(w/2) and 2(w/2)+1 of group
-    // w % 2; one: share w of it
-    const int a0 = __builtin_amdgcn_readfirstlane(sAlive[0]);
-    const int a1 = __builtin_amdgcn_readfirstlane(sAlive[1]);
-    if (a0 + a1 == 0) break;
-    {
-      const bool both = (a0 + a1) == 2;
-      const int g = both ? (w & 1) : (a0 ? 0 : 1);
-      const int sh0 = both ? 2 * (w >> 1) : w;
-      const int nsh = both ? 2 : 1;
-      const double xb = sP[g][0][lane], yb = sP[g][1][lane], zb = sP[g][2][lane];
-      for (int k = 0; k < nsh; ++k) {
-        const int share = sh0 + k;
-        double gx, gy, gz;
-        if constexpr (PREC == 1)
-          grav_split<PAIR_WAVES>(p.field, share, xb, yb, zb, gx, gy, gz);
-        else
-          grav_fp64<PAIR_WAVES>(p.field, share, xb, yb, zb, gx, gy, gz);
-        sPart[g][share][0][lane] = gx;
-        sPart[g][share][1][lane] = gy;
-        sPart[g][share][2][lane] = gz;
-      }
-    }
-    __syncthreads();
+    if (!pair_shares<PREC>(p, w, lane, sP, sPart, sAlive)) break;
+    __syncthreads();  // B
     // ------------------------------------------------------------ C
     // the shares in share order, then the rest of accel_eval's arithmetic,
     // so each acceleration equals the fixed-G kernel's at G = 8
-    if (leader && L.alive) {
+    if (L.alive) {
       double gx = sPart[grp][0][0][lane], gy = sPart[grp][0][1][lane], gz = sPart[grp][0][2][lane];
 #pragma unroll
       for (int s = 1; s < PAIR_WAVES; ++s) {
