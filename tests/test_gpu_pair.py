@@ -178,3 +178,34 @@ def test_automatic_launch_takes_the_pair_kernel_where_it_measured_faster():
     with bqp_amd.Propagator(f70, method="rk87", rtol=1e-12) as p:
         p.propagate_batch_final(w2["y0"], 0.0, np.full(98304, 60.0))
         assert p.last_launch()["kernel"] == "fixed" and p.last_launch()["waves_per_group"] == 4
+
+
+def test_automatic_pair_launch_under_graph_replay():
+    """The automatic pair launch (degree 70, one round of 4-wave groups)
+    captured in a HIP graph and replayed, with other device work and
+    synchronisation between replays, gives the direct launch's bits."""
+    from bqp_amd.device import DeviceArray, synchronize
+
+    n = 40_000
+    w = inputs.workload("w1", n=n, seed=46)
+    announce("automatic pair launch under hipGraph replay: degree 70, 40,000 rows x 10 min, drag")
+    y0 = DeviceArray.from_numpy(w["y0"])
+    t0 = DeviceArray.from_numpy(np.zeros(n))
+    tfd = DeviceArray.from_numpy(np.full(n, 600.0))
+    bc = DeviceArray.from_numpy(w["bc"])
+    pos = np.random.default_rng(1).normal(size=(4096, 3)) * 7e6
+    with bqp_amd.Propagator(bqp_amd.kaula(70), method="rk87", rtol=1e-12, drag="vallado") as p:
+        ref, sref, _ = p.propagate_batch_final_dev(y0, t0, tfd, bc)
+        assert p.last_launch()["kernel"] == "pair"
+        ref, sref = ref.numpy(), sref.numpy()
+        p.set_launch(use_graph=1)
+        yf = DeviceArray((n, 6))
+        st = DeviceArray((n,), np.int32)
+        for rep in range(3):
+            yf.copy_from(np.zeros((n, 6)))
+            p.gravity_accel(pos)  # other device work between replays
+            synchronize()
+            out, s, info = p.propagate_batch_final_dev(y0, t0, tfd, bc, yf, st)
+            assert p.last_launch()["kernel"] == "pair"
+            assert np.array_equal(out.numpy(), ref), rep
+            assert np.array_equal(s.numpy(), sref) and info["rows_ok"] == n
