@@ -112,32 +112,9 @@ constexpr int SECT_MAX = 208;
 static __constant__ double c_sect[SECT_MAX];
 static __constant__ float c_sectf[SECT_MAX];
 
-// One recursion step of a column in the rescaled form: U(n) from U(n-1),
-// U(n-2), then the contributions of entry e (11 FP64 operations).  Writes
-// the new value over U(n-2)'s register so a pair of steps needs no moves.
-#define BQP_COL_STEP(E, VOUT, VPREV, WOUT, WPREV) \
-  {                                               \
-    const double C_ = (E).g * rho;                \
-    VOUT = fma(zt, VPREV, -(C_ * VOUT));          \
-    WOUT = fma(zt, WPREV, -(C_ * WOUT));          \
-    sx = fma((E).xv, VOUT, sx);                   \
-    tx = fma((E).xw, WOUT, tx);                   \
-    sy = fma((E).yv, VOUT, sy);                   \
-    ty = fma((E).yw, WOUT, ty);                   \
-    sz = fma((E).zv, VOUT, sz);                   \
-    tz = fma((E).zw, WOUT, tz);                   \
-  }
-// Contributions of a column's sectoral entry (V, W given).
-#define BQP_SECT_TERM(E, V, W) \
-  {                            \
-    sx = fma((E).xv, V, sx);   \
-    tx = fma((E).xw, W, tx);   \
-    sy = fma((E).yv, V, sy);   \
-    ty = fma((E).yw, W, ty);   \
-    sz = fma((E).zv, V, sz);   \
-    tz = fma((E).zw, W, tz);   \
-  }
-// Column 0 (W = 0): the V half only.
+// One recursion step of column 0 (W = 0) in the rescaled form: U(n) from
+// U(n-1), U(n-2), then the entry's three terms.  Writes the new value over
+// U(n-2)'s register so a pair of steps needs no moves.
 #define BQP_COL0_STEP(E, VOUT, VPREV)    \
   {                                      \
     const double C_ = (E).g * rho;       \
@@ -151,9 +128,10 @@ static __constant__ float c_sectf[SECT_MAX];
 // recursion in n with the same coefficients and start from V(j-1,j) =
 // W(j-1,j) = 0, so V(n,j) = Q(n) V(j,j) and W(n,j) = Q(n) W(j,j), where
 // Q(j) = 1, Q(j-1) = 0 and Q(n) = zt Q(n-1) - g_n rho Q(n-2) (the rescaled
-// recursion above, run once instead of twice).  The column's six sums run
-// over Q alone and are multiplied by the sectoral value (V(j,j), W(j,j)) when
-// the column ends: 9 FP64 operations per entry instead of 11.
+// recursion, run once per column instead of once each for V and W).  The
+// column's six sums run over Q alone and are multiplied by the sectoral
+// value (V(j,j), W(j,j)) when the column ends: 9 FP64 operations per entry
+// instead of 11.
 #define BQP_QSTEP(E, QOUT, QPREV)        \
   {                                      \
     const double C_ = (E).g * rho;       \
@@ -403,8 +381,8 @@ __device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, d
   const double rr = 1.0 / r2;
   const double xt = x * R * rr, yt = y * R * rr, zt = z * R * rr, rho = R * R * rr;
   const double vs0 = R / sqrt(r2);
-  // V-term and W-term sums are kept apart (two accumulation chains per
-  // component, half the dependent-FMA latency per entry)
+  // column sums: V-coefficient terms in s, W-coefficient terms in t (six
+  // independent accumulation chains per column)
   double sx = 0.0, sy = 0.0, sz = 0.0, tx = 0.0, ty = 0.0, tz = 0.0;
   const int top = f.nmax + 1;
   CEntryL e = (CEntryL)f.tabl;
@@ -513,7 +491,7 @@ __device__ __forceinline__ void grav_fp64(const FieldView& f, int g, double x, d
 }
 
 // FP32 recursion step of the split form ((a, b) entries); the new value
-// overwrites the n-2 register as in BQP_COL_STEP.
+// overwrites the n-2 register as in BQP_QSTEP.
 #define BQP_COL_STEP_F(E, VOUT, VPREV, WOUT, WPREV) \
   {                                                 \
     const float A_ = (E).a * ztf, B_ = (E).b * rhof; \
@@ -683,20 +661,8 @@ __device__ __forceinline__ void grav_split_ab(const FieldView& f, int g, double 
 // V/s_n stays above 1e-28, far from FP32's 1.2e-38 normal minimum).
 constexpr int SPLIT_RESCALED_MAX_N = 90;
 
-// FP32 recursion step of the rescaled table (as BQP_COL_STEP) into the FP32
-// column partial sums q (V terms) and u (W terms).
-#define BQP_COL_STEP_LF(E, VOUT, VPREV, WOUT, WPREV) \
-  {                                                  \
-    const float C_ = (E).g * rhof;                   \
-    VOUT = fmaf(ztf, VPREV, -(C_ * VOUT));           \
-    WOUT = fmaf(ztf, WPREV, -(C_ * WOUT));           \
-    qx = fmaf((E).xv, VOUT, qx);                     \
-    ux = fmaf((E).xw, WOUT, ux);                     \
-    qy = fmaf((E).yv, VOUT, qy);                     \
-    uy = fmaf((E).yw, WOUT, uy);                     \
-    qz = fmaf((E).zv, VOUT, qz);                     \
-    uz = fmaf((E).zw, WOUT, uz);                     \
-  }
+// FP32 column-0 step of the rescaled table (as BQP_COL0_STEP) into the FP32
+// column partial sums q.
 #define BQP_COL0_STEP_LF(E, VOUT, VPREV)   \
   {                                        \
     const float C_ = (E).g * rhof;         \
