@@ -1,4 +1,4 @@
-"""GPU tests of the ragged-batch configuration at its full size and of the
+"""GPU tests of the ragged-batch workload at its full size and of the
 launch-shape autotuner.
 
 * The full 262,144-row ragged batch (60% LEO, 25% MEO, 15% GEO; durations
@@ -25,13 +25,13 @@ pytestmark = pytest.mark.gpu
 DAY = 86400.0
 
 
-def test_config2_full_batch_full_durations_vs_oracle(orc):
+def test_ragged_full_batch_full_durations_vs_oracle(orc):
     n = 262144
     w = inputs.workload("w2", n=n)
     tf = w["tf"]
     order = np.argsort(tf)
     pick = np.unique(np.concatenate([order[::4096], order[-8:]]))
-    announce(f"config 2: full ragged batch {n} rows, degree 20, rtol 1e-11; {pick.size} rows at full duration vs oracle")
+    announce(f"ragged workload: full batch {n} rows, degree 20, rtol 1e-11; {pick.size} rows at full duration vs oracle")
     field = bqp_amd.kaula(20)
     with bqp_amd.Propagator(field, method="rk87", rtol=1e-11) as p:
         p.set_launch(waves_per_group=8, sort_by_cost=1)
@@ -48,7 +48,7 @@ def test_config2_full_batch_full_durations_vs_oracle(orc):
     assert tf[pick].max() > 29.9 * DAY and tf[pick].min() < 1.01 * 3600.0
     assert steps.max() > 100 * steps.min()
     err = pos_err(yg[pick], yo)
-    announce(f"config 2 vs oracle: max {err.max():.3e} m, median {np.median(err):.3e} m; "
+    announce(f"ragged workload vs oracle: max {err.max():.3e} m, median {np.median(err):.3e} m; "
              f"oracle steps {steps.min()}..{steps.max()}; kernel {info['kernel_ms']:.1f} ms")
     assert err.max() < 1e-2
 

@@ -1,13 +1,13 @@
-"""GPU tests at the shapes of the learned-surrogate configuration and the
-population configuration.
+"""GPU tests at the shapes of the learned-surrogate workload and the
+population workload.
 
-Config 3 (learned surrogate): a strided sample (every 512th row, 128 rows)
+Learned-surrogate workload: a strided sample (every 512th row, 128 rows)
 of the 65,536-row LEO workload, 7 days, degree/order 70, Vallado drag,
 RK8(7)13M rtol 1e-12; the network trained as the bench trains it (10M
 device samples of the degree-70 residual above degree 4, 3,000 Adam steps).
 * FP64 GPU form against the oracle on 8 of the rows over one day (bar: the
   oracle's own one-ulp spread, as for every drag test);
-* BF16 and FP8 networks at the configuration's check bound (1e-9 m/s^2):
+* BF16 and FP8 networks at the workload's check bound (1e-9 m/s^2):
   every row fails its first check and the results equal the FP64 form
   bit for bit;
 * BF16 and FP8 networks at a bound set to the median of the network's error
@@ -16,7 +16,7 @@ device samples of the degree-70 residual above degree 4, 3,000 Adam steps).
   bit for bit, and the others stay finite (their distance from FP64 after 7
   days is reported, not bounded: no network here meets the 1 m target).
 
-Config 4 (population evaluation): one generation of 4,096 candidates
+Population workload: one generation of 4,096 candidates
 (seeded LEO departures, dv U(-50, 50) m/s per axis, arcs U(0.5, 10) d),
 degree/order 50, Vallado drag, RK8(7)13M rtol 1e-12, with the reference's
 ground event (100 km above the WGS84 ellipsoid: candidates whose perigee
@@ -61,14 +61,14 @@ def _train(prop, seed=0):
     return sur, feat
 
 
-def test_config3_surrogate_on_strided_sample(orc):
+def test_surrogate_workload_strided_sample(orc):
     w = inputs.workload("w1", n=65536)
     idx = np.arange(0, 65536, 512)
     n = idx.size
     y0, bc = w["y0"][idx], w["bc"][idx]
     field = bqp_amd.kaula(70)
     tf = np.full(n, 7 * DAY)
-    announce("config 3: FP64 form, 128 strided rows x 7 d, N=70, drag; 8 rows x 1 d vs oracle")
+    announce("surrogate workload: FP64 form, 128 strided rows x 7 d, N=70, drag; 8 rows x 1 d vs oracle")
     prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado")
     yref, sref, iref = prop.propagate_batch_final(y0, 0.0, tf, bc)
     assert np.all(sref == 0)
@@ -76,10 +76,10 @@ def test_config3_surrogate_on_strided_sample(orc):
     _, model, integ = oracle_pair(orc, field, method="rk87", rtol=1e-12, drag="vallado")
     yo, spread = _ulp_spread(orc, model, integ, y0[:8], DAY, bc[:8])
     err = pos_err(y1, yo)
-    announce(f"config 3 FP64 vs oracle (1 d): max {err.max():.3e} m; oracle 1-ulp spread {spread.max():.3e} m")
+    announce(f"surrogate workload FP64 vs oracle (1 d): max {err.max():.3e} m; oracle 1-ulp spread {spread.max():.3e} m")
     assert np.all(s1 == 0) and err.max() <= 3.0 * spread.max() + 1e-2
 
-    announce("config 3: training the network (10M samples, 3,000 steps)")
+    announce("surrogate workload: training the network (10M samples, 3,000 steps)")
     sur, feat = _train(prop)
     # network error at the rows' start positions (body frame = inertial at t = 0)
     pos = y0[:, :3]
@@ -92,9 +92,9 @@ def test_config3_surrogate_on_strided_sample(orc):
         sur.pack(fmt, calib=feat.rows(0, 65536))
         net, _ = sur.infer(F, fmt)
         e0 = np.linalg.norm(low + net.numpy() * sur.out_scale - full, axis=1)
-        announce(f"config 3 {fmt}: network error at the start positions median {np.median(e0):.3e}, "
+        announce(f"surrogate workload {fmt}: network error at the start positions median {np.median(e0):.3e}, "
                  f"min {e0.min():.3e} m/s^2")
-        # the configuration's bound: no row passes, every row is the FP64 row
+        # the workload's check bound: no row passes, every row is the FP64 row
         sur.set_check(100, 1e-9)
         y, s, info = prop.propagate_batch_final(y0, 0.0, tf, bc)
         st = sur.last_stats()
@@ -108,7 +108,7 @@ def test_config3_surrogate_on_strided_sample(orc):
         frac = st["net_evals"] / info["evals"]
         first_fail = e0 > thr
         d = pos_err(y, yref)
-        announce(f"config 3 {fmt} at bound {thr:.3e}: network on {100 * frac:.1f}% of {info['evals']} evaluations, "
+        announce(f"surrogate workload {fmt} at bound {thr:.3e}: network on {100 * frac:.1f}% of {info['evals']} evaluations, "
                  f"{int((~first_fail).sum())}/{n} rows pass the first check, {st['check_failures']} check failures; "
                  f"7-day distance from FP64 on those rows median {np.median(d[~first_fail]):.3e} m, "
                  f"max {d[~first_fail].max():.3e} m")
@@ -123,17 +123,17 @@ def test_config3_surrogate_on_strided_sample(orc):
     prop.close()
 
 
-def test_config4_one_generation(orc):
+def test_population_one_generation(orc):
     w = inputs.workload("w4", n=4096)
     y0, tf, bc = w["y0"], w["tf"], w["bc"]
     field = bqp_amd.kaula(50)
-    announce("config 4: one generation, 4,096 candidates, arcs 0.5-10 d, N=50, drag, ground event (FP64 form)")
+    announce("population workload: one generation, 4,096 candidates, arcs 0.5-10 d, N=50, drag, ground event (FP64 form)")
     ground = dict(r_min=bqp_amd.R_EARTH + bqp_amd.GROUND_ALTITUDE_M, ground_flattening=bqp_amd.WGS84_FLATTENING)
     prop = bqp_amd.Propagator(field, method="rk87", rtol=1e-12, drag="vallado", **ground)
     yref, sref, info = prop.propagate_batch_final(y0, 0.0, tf, bc)
     reentered = np.flatnonzero(sref == 6)
     assert np.all((sref == 0) | (sref == 6)) and 0 < reentered.size < 100
-    announce(f"config 4 FP64: {np.sum(tf) / DAY:.0f} orbit-days, kernel {info['kernel_ms']:.0f} ms, "
+    announce(f"population workload FP64: {np.sum(tf) / DAY:.0f} orbit-days, kernel {info['kernel_ms']:.0f} ms, "
              f"{reentered.size} candidates re-enter, evals per row max {info['evals_per_row'].max()}")
     idx = np.union1d(np.arange(0, 4096, 512), reentered)
     _, model, integ = oracle_pair(orc, field, method="rk87", rtol=1e-12, drag="vallado", **ground)
@@ -145,12 +145,12 @@ def test_config4_one_generation(orc):
     yb, _, _, _ = orc.propagate_batch(model, integ, yp, 0.0, tf[ok], bc[ok])
     spread = pos_err(yo[so == 0], yb)
     err = pos_err(yref[ok], yo[so == 0])
-    announce(f"config 4 FP64 vs oracle on {ok.size} finishing candidates: max {err.max():.3e} m, median "
+    announce(f"population workload FP64 vs oracle on {ok.size} finishing candidates: max {err.max():.3e} m, median "
              f"{np.median(err):.3e} m; oracle 1-ulp spread max {spread.max():.3e} m; statuses equal on "
              f"{idx.size} candidates")
     assert err.max() <= 3.0 * spread.max() + 1e-2
 
-    announce("config 4: training the network (N=50), checked BF16 and FP8 forms vs FP64")
+    announce("population workload: training the network (N=50), checked BF16 and FP8 forms vs FP64")
     sur, feat = _train(prop, seed=3)
     prop.reconfigure(precision="surrogate")
     for fmt in ("bf16", "fp8"):
@@ -158,7 +158,7 @@ def test_config4_one_generation(orc):
         sur.set_check(100, 1e-9)
         y, s, inf = prop.propagate_batch_final(y0, 0.0, tf, bc)
         st = sur.last_stats()
-        announce(f"config 4 {fmt}: {st}, kernel {inf['kernel_ms']:.0f} ms")
+        announce(f"population workload {fmt}: {st}, kernel {inf['kernel_ms']:.0f} ms")
         assert np.array_equal(s, sref)
         assert st["net_evals"] == 0 and st["check_failures"] == 4096
         assert np.array_equal(y, yref, equal_nan=True)
